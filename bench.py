@@ -1,0 +1,204 @@
+"""Benchmark: homogenized RVE tangents per second at 30^3 hex8 on one MI355X.
+
+A step is one pass of the hot path over one batch: B seeded 30^3 two-phase
+RVEs (fiber radius ~ U[0.15, 0.35]) are admitted, assembled on the GPU, their
+six unit-strain load cases solved as a 6-RHS block by FP32 Jacobi-PCG inside
+FP64 iterative refinement, and their 6x6 tangents averaged in FP64.
+
+The JSON line also carries:
+  roofline      the batched ELL SpMV kernel, timed with HIP events
+  cpu_baseline  the FP64 oracle (one core, one RVE at a time) on a sample
+  plain_fp64    the plain FP64 single-RVE GPU path (one load case at a time)
+  accuracy      max normwise relative tangent error vs the FP64 oracle
+  serving       p50 / p99 latency of a short FE^2 request stream
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+import numpy as np  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "RVE tangents/sec at 30³ hex8 (rel err≤1e-8); p99 latency; SpMV HBM GB/s"
+
+
+def log(msg):
+    print("[bench] " + msg, file=sys.stderr, flush=True)
+
+
+def run_batch(eng, specs, mm, warm=False):
+    for i, s in enumerate(specs):
+        rc = eng.submit(i + 1, s, mm.unit_loads(), mm.MM_TANGENT_UNIT,
+                        warm_key=(i + 1) if warm else 0)
+        if rc:
+            raise RuntimeError("submit failed: %d" % rc)
+    eng.drain()
+    out = {r["id"]: r for r in eng.poll(len(specs) + 8)}
+    return [out[i + 1] for i in range(len(specs))]
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--batch", type=int, default=128)
+    ap.add_argument("--n", type=int, default=30)
+    ap.add_argument("--slab", type=int, default=256)
+    ap.add_argument("--inner-rtol", type=float, default=1e-5)
+    ap.add_argument("--spl", type=int, default=16, help="PCG steps per graph launch")
+    ap.add_argument("--cpu-sample", type=int, default=2, help="RVE tangents timed on the CPU")
+    ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--serve", type=int, default=1024, help="requests in the serving sample")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build measures one GPU; multi-GPU runs are not part of it "
+              "(RVEs shard across GPUs without communication, see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    import madmax_amd as mm
+    from madmax_amd import configs
+
+    B, n = args.batch, args.n
+    specs = [configs.c2_spec(2, i, n=n) for i in range(B)]
+    rows = 3 * (n + 1) ** 3
+    per_rve = rows * 900 + (1 << 20)
+    budget = int(per_rve * B * 1.3) + (256 << 20)
+    log("batch %d x %d^3 (%d rows each), arena %.1f GB" % (B, n, rows, budget / 1e9))
+    eng = mm.Engine(nrhs=6, precision=mm.MM_PREC_MIXED, hbm_budget=budget, max_rves=B + 8,
+                    slab_rows=args.slab, inner_rtol=args.inner_rtol, rtol=1e-10,
+                    steps_per_launch=args.spl, use_graph=True)
+    for w in range(args.warmup):
+        t = time.perf_counter()
+        run_batch(eng, specs, mm)
+        log("warmup %d: %.3f s" % (w, time.perf_counter() - t))
+    t0 = time.perf_counter()
+    results = None
+    for k in range(args.steps):
+        results = run_batch(eng, specs, mm)
+    elapsed = time.perf_counter() - t0
+    ms_per_step = 1e3 * elapsed / args.steps
+    value = B * args.steps / elapsed
+    its = [r["pcg_its"] for r in results]
+    sweeps = [r["sweeps"] for r in results]
+    status_ok = all(r["status"] == 0 for r in results)
+    log("timed: %.3f s for %d steps -> %.1f tangents/s (its %d..%d, sweeps %d..%d)"
+        % (elapsed, args.steps, value, min(its), max(its), min(sweeps), max(sweeps)))
+
+    # roofline of the batched ELL SpMV on the resident batch
+    run_batch(eng, specs, mm, warm=True)
+    spmv_ms, spmv_bytes = eng.time_spmv(reps=50)
+    achieved = spmv_bytes / (spmv_ms * 1e-3) / 1e9
+    log("spmv: %.3f ms, %.3f GB/launch -> %.0f GB/s" % (spmv_ms, spmv_bytes / 1e9, achieved))
+    eng.close()
+
+    # plain FP64 single-RVE GPU path: one load case at a time, host check per iteration
+    plain = None
+    if not args.no_plain:
+        pe = mm.Engine(nrhs=1, precision=mm.MM_PREC_FP64, hbm_budget=per_rve * 2 + (64 << 20),
+                       max_rves=2, slab_rows=256, inner_rtol=1e-10, rtol=0.0, abs_tol=1e-4,
+                       max_sweeps=2, sync_every_step=True, use_graph=False)
+        nplain = 2
+        t = time.perf_counter()
+        cs = []
+        for j in range(nplain):
+            sig = []
+            for i, load in enumerate(mm.unit_loads()):
+                pe.submit(10 * j + i + 1, specs[j], [load], mm.MM_TANGENT_NONE, warm_key=j + 1)
+                pe.drain()
+                sig.append(pe.poll()[0]["stress"][0] / 1e-3)
+            cs.append(np.array(sig).T)
+        tp = time.perf_counter() - t
+        pe.close()
+        plain = dict(value=nplain / tp, unit="RVE tangents/s", n=nplain)
+        log("plain fp64: %.2f tangents/s" % plain["value"])
+
+    # CPU baseline: the FP64 oracle, one core, one RVE at a time (bounded sample)
+    cpu = None
+    acc = None
+    if not args.no_cpu:
+        from oracle import pyoracle as po
+        errs = []
+        t = time.perf_counter()
+        for j in range(args.cpu_sample):
+            s = specs[j]
+            m = po.mesh(3, n, n, n)
+            ro = po.Rve(3, n, po.phase_map(m, mm.spec_fibers(s)), [s.young[0], s.young[1]],
+                        [s.poisson[0], s.poisson[1]])
+            c_ref, _ = ro.c_tangent_unit(1e-3, 1e-10)
+            errs.append(float(np.max(np.abs(results[j]["c_tangent"] - c_ref)) / np.max(np.abs(c_ref))))
+            if plain is not None and j < len(cs):
+                plain.setdefault("err", []).append(
+                    float(np.max(np.abs(cs[j] - c_ref)) / np.max(np.abs(c_ref))))
+        tc = time.perf_counter() - t
+        cpu = dict(value=args.cpu_sample / tc, unit="RVE tangents/s", cores=1, kind="port",
+                   sample="%d seeded %d^3 RVE tangents (assembly + 6 FP64 Jacobi-PCG solves to 1e-10 "
+                          "+ averaging), oracle/oracle.c gcc -O3, one thread" % (args.cpu_sample, n))
+        acc = max(errs)
+        log("cpu oracle: %.3f tangents/s, max tangent err %.2e" % (cpu["value"], acc))
+
+    # serving sample (FE^2 stream, mixed RVE sizes, Poisson arrivals)
+    serving = None
+    if args.serve > 0:
+        stream = configs.c5_stream(count=args.serve, rate=4000.0)
+        se = mm.Engine(nrhs=7, precision=mm.MM_PREC_MIXED, hbm_budget=8 << 30, max_rves=512,
+                       slab_rows=args.slab, inner_rtol=args.inner_rtol, steps_per_launch=8)
+        se.start()
+        t0s = time.perf_counter()
+        got = []
+        for i, rq in enumerate(stream):
+            while time.perf_counter() - t0s < rq["arrival"]:
+                got.extend(se.poll())
+                time.sleep(0.0002)
+            loads = [np.array([1e-3 if a == b else 0.0 for b in range(6)]) for a in range(6)]
+            se.submit(i + 1, rq["spec"], loads + [rq["strain"]], mm.MM_TANGENT_UNIT)
+        while len(got) < len(stream) and time.perf_counter() - t0s < 300:
+            got.extend(se.poll())
+            time.sleep(0.0005)
+        ts = time.perf_counter() - t0s
+        se.stop()
+        se.close()
+        lat = np.array([(r["t_done"] - r["t_submit"]) * 1e3 for r in got])
+        serving = dict(requests=len(got), tangents_per_s=len(got) / ts,
+                       p50_ms=float(np.percentile(lat, 50)), p99_ms=float(np.percentile(lat, 99)),
+                       all_ok=all(r["status"] == 0 for r in got))
+        log("serving: %s" % serving)
+
+    line = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "RVE tangents/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 PCG in f64 refinement",
+        "data": "synthetic (seeded two-phase fiber RVEs)",
+        "config": {"workload": "configs[1]: %d x %d^3 hex8 RVE tangents per step, 6-RHS block, "
+                               "FP32 PCG + FP64 IR" % (B, n), "rve_elements": n ** 3,
+                   "batch": B, "slab_rows": args.slab, "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "k_spmv_time<float,6> (batched ELL SpMV, %d resident RVEs)" % B,
+                     "avg_ms": spmv_ms, "bytes_per_launch": spmv_bytes},
+        "cpu_baseline": cpu,
+        "plain_fp64_gpu": plain,
+        "speedup_vs_plain": (value / plain["value"]) if plain else None,
+        "accuracy": {"max_rel_err_tangent": acc, "all_converged": status_ok,
+                     "pcg_its": [min(its), max(its)], "ir_sweeps": [min(sweeps), max(sweeps)]},
+        "serving": serving,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,521 @@
+// ell_device.hip -- device mirror of the reference ELL matrix and the GPU
+// compute entry points of include/ell.h (ell_mvp, ell_solve_jacobi,
+// ell_solve_pcg).
+//
+// HBM layout: rows are grouped into slabs of S rows (S a multiple of 64);
+// inside a slab the slots are column-major, element (row, j) at
+// slab*S*nnz + j*S + row%S, so a wavefront reading slot j of 64 consecutive
+// rows issues one coalesced 256-byte (int32) / 512-byte (FP64) access.
+// Padding rows of the last slab hold cols = -1, vals = 0.
+//
+// Bit-exactness: the SpMV accumulates each row in ascending slot order,
+// stops at the first -1 and keeps multiply and add separate, which is the
+// arithmetic of src/ell.c:127-141 (the reference is built without FMA).
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+#include <vector>
+
+#include "ell.h"
+#include "mm_device.h"
+#include "mm_internal.h"
+
+using namespace mm;
+
+namespace {
+
+struct IterState {
+  double bnorm, rnorm, rz, alpha, beta, err;
+  int its, done, max_its, pbuf;
+  double rtol, min_err;
+  int cnt0, cnt1, pad0, pad1;
+};
+
+__device__ inline double row_dot_exact(const int *__restrict__ cols, const double *__restrict__ vals,
+                                       long long slab_base, int S, int nnz, int lane,
+                                       const double *__restrict__ x)
+{
+#pragma clang fp contract(off)
+  double acc = 0.0;
+  for (int j = 0; j < nnz; j++) {
+    const int c = cols[slab_base + (long long)j * S + lane];
+    if (c == -1) break;
+    acc = acc + vals[slab_base + (long long)j * S + lane] * x[c];
+  }
+  return acc;
+}
+
+// y = A x, one lane per row (src/ell.c:127-141)
+__global__ void k_spmv64_exact(const int *__restrict__ cols, const double *__restrict__ vals,
+                               int nrow, int nnz, int S, const double *__restrict__ x,
+                               double *__restrict__ y)
+{
+  const int lane = threadIdx.x;
+  const long long row = (long long)blockIdx.x * S + lane;
+  if (row >= nrow) return;
+  const long long base = (long long)blockIdx.x * S * nnz;
+  y[row] = row_dot_exact(cols, vals, base, S, nnz, lane, x);
+}
+
+// ---- stationary Jacobi (src/ell.c:143-184) --------------------------------
+// k_jac_res: r = A x - b and ||r||^2; the last block decides convergence.
+__global__ void k_jac_res(const int *__restrict__ cols, const double *__restrict__ vals, int nrow,
+                          int nnz, int S, const double *__restrict__ x, const double *__restrict__ b,
+                          double *__restrict__ r, double *__restrict__ part, IterState *st)
+{
+#pragma clang fp contract(off)
+  __shared__ double lds[16];
+  if (st->done) return;
+  if (st->its >= st->max_its) {  // loop condition of the reference
+    if (blockIdx.x == 0 && threadIdx.x == 0) st->done = 1;
+    return;
+  }
+  const int lane = threadIdx.x;
+  const long long row = (long long)blockIdx.x * S + lane;
+  double v[1] = {0.0};
+  if (row < nrow) {
+    const long long base = (long long)blockIdx.x * S * nnz;
+    double ri = row_dot_exact(cols, vals, base, S, nnz, lane, x);
+    ri = ri - b[row];
+    r[row] = ri;
+    v[0] = ri * ri;
+  }
+  block_sum<1>(v, lds);
+  if (threadIdx.x == 0) part[blockIdx.x] = v[0];
+  if (arrive_last(&st->cnt0, gridDim.x) && threadIdx.x == 0) {
+    double s = 0.0;
+    for (int i = 0; i < (int)gridDim.x; i++) s += load_coherent(&part[i]);
+    st->err = sqrt(s);
+    if (st->err < st->min_err) st->done = 1;
+    else st->its += 1;
+  }
+}
+
+// k_jac_upd: x -= dinv * r
+__global__ void k_jac_upd(int nrow, int S, const double *__restrict__ dinv,
+                          const double *__restrict__ r, double *__restrict__ x, const IterState *st)
+{
+#pragma clang fp contract(off)
+  if (st->done) return;
+  const long long row = (long long)blockIdx.x * S + threadIdx.x;
+  if (row >= nrow) return;
+  const double e = dinv[row] * r[row];
+  x[row] = x[row] - e;
+}
+
+// ---- Jacobi-preconditioned CG in FP64 ---------------------------------------
+// init: r = b - A x, z = dinv r, ||b||, ||r||, r.z
+__global__ void k_pcg_init(const int *__restrict__ cols, const double *__restrict__ vals, int nrow,
+                           int nnz, int S, const double *__restrict__ x, const double *__restrict__ b,
+                           const double *__restrict__ dinv, double *__restrict__ r,
+                           double *__restrict__ z, double *__restrict__ p0, double *__restrict__ p1,
+                           double *__restrict__ part, IterState *st)
+{
+  __shared__ double lds[48];
+  const int lane = threadIdx.x;
+  const long long row = (long long)blockIdx.x * S + lane;
+  double v[3] = {0.0, 0.0, 0.0};
+  if (row < nrow) {
+    const long long base = (long long)blockIdx.x * S * nnz;
+    const double ax = row_dot_exact(cols, vals, base, S, nnz, lane, x);
+    const double bi = b[row];
+    const double ri = bi - ax;
+    const double zi = dinv[row] * ri;
+    r[row] = ri;
+    z[row] = zi;
+    p0[row] = 0.0;
+    p1[row] = 0.0;
+    v[0] = bi * bi;
+    v[1] = ri * ri;
+    v[2] = ri * zi;
+  }
+  block_sum<3>(v, lds);
+  if (threadIdx.x == 0)
+    for (int i = 0; i < 3; i++) part[blockIdx.x * 4 + i] = v[i];
+  if (arrive_last(&st->cnt0, gridDim.x) && threadIdx.x == 0) {
+    double s[3] = {0.0, 0.0, 0.0};
+    for (int k = 0; k < (int)gridDim.x; k++)
+      for (int i = 0; i < 3; i++) s[i] += load_coherent(&part[k * 4 + i]);
+    st->bnorm = sqrt(s[0]);
+    st->rnorm = sqrt(s[1]);
+    st->rz = s[2];
+    st->beta = 0.0;
+    st->its = 0;
+    st->pbuf = 0;
+    st->done = (st->bnorm == 0.0 || st->rnorm <= st->rtol * st->bnorm) ? 1 : 0;
+  }
+}
+
+// q = A p with p = z + beta p_old formed on the fly (both for the own rows,
+// which are stored, and for every gathered column), then alpha = r.z / p.q
+__global__ void k_pcg_spmv(const int *__restrict__ cols, const double *__restrict__ vals, int nrow,
+                           int nnz, int S, const double *__restrict__ z, double *__restrict__ pA,
+                           double *__restrict__ pB, double *__restrict__ q,
+                           double *__restrict__ part, IterState *st)
+{
+  __shared__ double lds[16];
+  if (st->done) return;
+  const double beta = st->beta;
+  const double *pold = st->pbuf ? pB : pA;
+  double *pnew = st->pbuf ? pA : pB;
+  const int lane = threadIdx.x;
+  const long long row = (long long)blockIdx.x * S + lane;
+  double v[1] = {0.0};
+  if (row < nrow) {
+    const long long base = (long long)blockIdx.x * S * nnz;
+    double acc = 0.0;
+    for (int j = 0; j < nnz; j++) {
+      const int c = cols[base + (long long)j * S + lane];
+      if (c == -1) break;
+      acc += vals[base + (long long)j * S + lane] * (z[c] + beta * pold[c]);
+    }
+    const double pi = z[row] + beta * pold[row];
+    pnew[row] = pi;
+    q[row] = acc;
+    v[0] = pi * acc;
+  }
+  block_sum<1>(v, lds);
+  if (threadIdx.x == 0) part[blockIdx.x * 4] = v[0];
+  if (arrive_last(&st->cnt0, gridDim.x) && threadIdx.x == 0) {
+    double s = 0.0;
+    for (int k = 0; k < (int)gridDim.x; k++) s += load_coherent(&part[k * 4]);
+    st->alpha = st->rz / s;
+    st->pbuf ^= 1;
+  }
+}
+
+// x += alpha p, r -= alpha q, z = dinv r, then r.r and r.z
+__global__ void k_pcg_update(int nrow, int S, const double *__restrict__ dinv,
+                             const double *__restrict__ pA, const double *__restrict__ pB,
+                             const double *__restrict__ q, double *__restrict__ x,
+                             double *__restrict__ r, double *__restrict__ z,
+                             double *__restrict__ part, IterState *st)
+{
+  __shared__ double lds[32];
+  if (st->done) return;
+  const double alpha = st->alpha;
+  const double *p = st->pbuf ? pB : pA;  // buffer written by k_pcg_spmv
+  const long long row = (long long)blockIdx.x * S + threadIdx.x;
+  double v[2] = {0.0, 0.0};
+  if (row < nrow) {
+    const double xi = x[row] + alpha * p[row];
+    const double ri = r[row] - alpha * q[row];
+    const double zi = dinv[row] * ri;
+    x[row] = xi;
+    r[row] = ri;
+    z[row] = zi;
+    v[0] = ri * ri;
+    v[1] = ri * zi;
+  }
+  block_sum<2>(v, lds);
+  if (threadIdx.x == 0) {
+    part[blockIdx.x * 4 + 1] = v[0];
+    part[blockIdx.x * 4 + 2] = v[1];
+  }
+  if (arrive_last(&st->cnt1, gridDim.x) && threadIdx.x == 0) {
+    double s[2] = {0.0, 0.0};
+    for (int k = 0; k < (int)gridDim.x; k++) {
+      s[0] += load_coherent(&part[k * 4 + 1]);
+      s[1] += load_coherent(&part[k * 4 + 2]);
+    }
+    st->its += 1;
+    st->rnorm = sqrt(s[0]);
+    if (st->rnorm <= st->rtol * st->bnorm || st->its >= st->max_its) {
+      st->done = 1;
+    } else {
+      st->beta = s[1] / st->rz;
+      st->rz = s[1];
+    }
+  }
+}
+
+}  // namespace
+
+struct ell_device_ {
+  int nrow, ncol, nnz, S, nslab;
+  int *cols;
+  double *vals, *dinv;
+  double *x, *b, *r, *z, *pA, *pB, *q, *part;
+  IterState *st;
+  hipStream_t stream;
+};
+
+static void host_diag_inv(const ell_matrix *m, std::vector<double> &dinv)
+{
+  dinv.resize(m->nrow);
+  for (int i = 0; i < m->nrow; i++) {
+    double d = 0.0;
+    if (i < m->ncol) {
+      const int *c = m->cols + (size_t)i * m->nnz;
+      for (int j = 0; j < m->nnz; j++) {
+        if (c[j] == -1) break;
+        if (c[j] == i) {
+          d = m->vals[(size_t)i * m->nnz + j];
+          break;
+        }
+      }
+    }
+    dinv[i] = 1 / d;  // like the reference, a missing diagonal gives inf
+  }
+}
+
+static int upload(ell_device *d, const ell_matrix *m)
+{
+  const size_t slots = (size_t)d->nslab * d->S * d->nnz;
+  std::vector<int> hc(slots, -1);
+  std::vector<double> hv(slots, 0.0);
+  for (int i = 0; i < m->nrow; i++) {
+    const size_t sb = (size_t)(i / d->S) * d->S * d->nnz, lane = i % d->S;
+    for (int j = 0; j < m->nnz; j++) {
+      hc[sb + (size_t)j * d->S + lane] = m->cols[(size_t)i * m->nnz + j];
+      hv[sb + (size_t)j * d->S + lane] = m->vals[(size_t)i * m->nnz + j];
+    }
+  }
+  std::vector<double> dinv;
+  host_diag_inv(m, dinv);
+  MM_HIP(hipMemcpyAsync(d->cols, hc.data(), slots * sizeof(int), hipMemcpyHostToDevice, d->stream));
+  MM_HIP(hipMemcpyAsync(d->vals, hv.data(), slots * sizeof(double), hipMemcpyHostToDevice, d->stream));
+  if (m->nrow)
+    MM_HIP(hipMemcpyAsync(d->dinv, dinv.data(), m->nrow * sizeof(double), hipMemcpyHostToDevice,
+                          d->stream));
+  MM_HIP(hipStreamSynchronize(d->stream));
+  return 0;
+}
+
+extern "C" int ell_device_create(const ell_matrix *m, int slab_rows, ell_device **out)
+{
+  if (m == NULL || out == NULL || m->cols == NULL || m->vals == NULL) return 1;
+  if (m->nrow < 0 || m->nnz < 0 || m->ncol < 0) return 2;
+  int S = slab_rows > 0 ? slab_rows : 256;
+  if (S % 64 != 0 || S > 1024) return 1;
+  ell_device *d = (ell_device *)calloc(1, sizeof(ell_device));
+  if (!d) return 2;
+  d->nrow = m->nrow;
+  d->ncol = m->ncol;
+  d->nnz = m->nnz;
+  d->S = S;
+  d->nslab = (m->nrow + S - 1) / S;
+  const size_t slots = (size_t)d->nslab * S * d->nnz;
+  const size_t nv = (size_t)(d->nslab ? d->nslab : 1) * S;
+  const size_t nvc = (size_t)(m->ncol > (int)nv ? m->ncol : nv);
+  int rc = 0;
+#define ALLOC(ptr, bytes)                                                   \
+  if (hipMalloc((void **)&(ptr), (bytes) ? (bytes) : 8) != hipSuccess) { \
+    rc = MM_ERR_HIP;                                                         \
+    goto fail;                                                               \
+  }
+  if (hipStreamCreateWithFlags(&d->stream, hipStreamNonBlocking) != hipSuccess) {
+    free(d);
+    return MM_ERR_HIP;
+  }
+  ALLOC(d->cols, slots * sizeof(int));
+  ALLOC(d->vals, slots * sizeof(double));
+  ALLOC(d->dinv, nv * sizeof(double));
+  ALLOC(d->x, nvc * sizeof(double));
+  ALLOC(d->b, nv * sizeof(double));
+  ALLOC(d->r, nv * sizeof(double));
+  ALLOC(d->z, nvc * sizeof(double));
+  ALLOC(d->pA, nvc * sizeof(double));
+  ALLOC(d->pB, nvc * sizeof(double));
+  ALLOC(d->q, nv * sizeof(double));
+  ALLOC(d->part, (size_t)(d->nslab + 1) * 4 * sizeof(double));
+  ALLOC(d->st, sizeof(IterState));
+#undef ALLOC
+  if (hipMemsetAsync(d->st, 0, sizeof(IterState), d->stream) != hipSuccess) {
+    rc = MM_ERR_HIP;
+    goto fail;
+  }
+  rc = upload(d, m);
+  if (rc) goto fail;
+  *out = d;
+  return 0;
+fail:
+  ell_device_destroy(d);
+  return rc;
+}
+
+extern "C" int ell_device_update(ell_device *d, const ell_matrix *m)
+{
+  if (!d || !m || m->nrow != d->nrow || m->nnz != d->nnz || m->ncol != d->ncol) return 1;
+  return upload(d, m);
+}
+
+extern "C" int ell_device_download(const ell_device *d, int *cols, double *vals)
+{
+  if (!d || !cols || !vals) return 1;
+  const size_t slots = (size_t)d->nslab * d->S * d->nnz;
+  std::vector<int> hc(slots);
+  std::vector<double> hv(slots);
+  MM_HIP(hipMemcpyAsync(hc.data(), d->cols, slots * sizeof(int), hipMemcpyDeviceToHost, d->stream));
+  MM_HIP(hipMemcpyAsync(hv.data(), d->vals, slots * sizeof(double), hipMemcpyDeviceToHost, d->stream));
+  MM_HIP(hipStreamSynchronize(d->stream));
+  for (int i = 0; i < d->nrow; i++) {
+    const size_t sb = (size_t)(i / d->S) * d->S * d->nnz, lane = i % d->S;
+    for (int j = 0; j < d->nnz; j++) {
+      cols[(size_t)i * d->nnz + j] = hc[sb + (size_t)j * d->S + lane];
+      vals[(size_t)i * d->nnz + j] = hv[sb + (size_t)j * d->S + lane];
+    }
+  }
+  return 0;
+}
+
+extern "C" int ell_device_mvp(ell_device *d, const double *x, double *y)
+{
+  if (!d || !x || !y) return 1;
+  if (d->nrow == 0) return 0;
+  MM_HIP(hipMemcpyAsync(d->x, x, d->ncol * sizeof(double), hipMemcpyHostToDevice, d->stream));
+  k_spmv64_exact<<<d->nslab, d->S, 0, d->stream>>>(d->cols, d->vals, d->nrow, d->nnz, d->S, d->x, d->q);
+  MM_HIP(hipGetLastError());
+  MM_HIP(hipMemcpyAsync(y, d->q, d->nrow * sizeof(double), hipMemcpyDeviceToHost, d->stream));
+  MM_HIP(hipStreamSynchronize(d->stream));
+  return 0;
+}
+
+static int set_state(ell_device *d, int max_its, double rtol, double min_err)
+{
+  IterState h;
+  memset(&h, 0, sizeof(h));
+  h.max_its = max_its;
+  h.rtol = rtol;
+  h.min_err = min_err;
+  MM_HIP(hipMemcpyAsync(d->st, &h, sizeof(h), hipMemcpyHostToDevice, d->stream));
+  return 0;
+}
+
+static int get_state(ell_device *d, IterState *h)
+{
+  MM_HIP(hipMemcpyAsync(h, d->st, sizeof(*h), hipMemcpyDeviceToHost, d->stream));
+  MM_HIP(hipStreamSynchronize(d->stream));
+  return 0;
+}
+
+extern "C" int ell_device_solve_jacobi(ell_device *d, ell_solver *solver, const double *b, double *x)
+{
+  if (!d || !b || !x) return 1;
+  if (d->nrow != d->ncol) return 1;
+  if (d->nrow == 0) return 0;
+  int rc = set_state(d, MAX_ITS, 0.0, MIN_ERR);
+  if (rc) return rc;
+  MM_HIP(hipMemcpyAsync(d->x, x, d->nrow * sizeof(double), hipMemcpyHostToDevice, d->stream));
+  MM_HIP(hipMemcpyAsync(d->b, b, d->nrow * sizeof(double), hipMemcpyHostToDevice, d->stream));
+  IterState h;
+  for (;;) {
+    for (int k = 0; k < 64; k++) {
+      k_jac_res<<<d->nslab, d->S, 0, d->stream>>>(d->cols, d->vals, d->nrow, d->nnz, d->S, d->x, d->b,
+                                                  d->r, d->part, d->st);
+      k_jac_upd<<<d->nslab, d->S, 0, d->stream>>>(d->nrow, d->S, d->dinv, d->r, d->x, d->st);
+    }
+    MM_HIP(hipGetLastError());
+    rc = get_state(d, &h);
+    if (rc) return rc;
+    if (h.done) break;
+  }
+  MM_HIP(hipMemcpyAsync(x, d->x, d->nrow * sizeof(double), hipMemcpyDeviceToHost, d->stream));
+  MM_HIP(hipStreamSynchronize(d->stream));
+  if (solver) {
+    solver->its = h.its;
+    solver->err = h.err;
+  }
+  return 0;
+}
+
+extern "C" int ell_device_solve_pcg(ell_device *d, ell_solver *solver, const double *b, double *x)
+{
+  if (!d || !b || !x) return 1;
+  if (d->nrow != d->ncol) return 1;
+  if (d->nrow == 0) return 0;
+  int max_its = (solver && solver->max_its > 0) ? solver->max_its : MAX_ITS;
+  double rtol = (solver && solver->min_err > 0) ? solver->min_err : MIN_ERR;
+  int rc = set_state(d, max_its, rtol, 0.0);
+  if (rc) return rc;
+  MM_HIP(hipMemcpyAsync(d->x, x, d->nrow * sizeof(double), hipMemcpyHostToDevice, d->stream));
+  MM_HIP(hipMemcpyAsync(d->b, b, d->nrow * sizeof(double), hipMemcpyHostToDevice, d->stream));
+  k_pcg_init<<<d->nslab, d->S, 0, d->stream>>>(d->cols, d->vals, d->nrow, d->nnz, d->S, d->x, d->b,
+                                               d->dinv, d->r, d->z, d->pA, d->pB, d->part, d->st);
+  MM_HIP(hipGetLastError());
+  IterState h;
+  rc = get_state(d, &h);
+  if (rc) return rc;
+  if (h.bnorm == 0.0) {  // zero right-hand side: the solution is zero
+    MM_HIP(hipMemsetAsync(d->x, 0, d->nrow * sizeof(double), d->stream));
+  }
+  while (!h.done) {
+    for (int k = 0; k < 32; k++) {
+      k_pcg_spmv<<<d->nslab, d->S, 0, d->stream>>>(d->cols, d->vals, d->nrow, d->nnz, d->S, d->z,
+                                                   d->pA, d->pB, d->q, d->part, d->st);
+      k_pcg_update<<<d->nslab, d->S, 0, d->stream>>>(d->nrow, d->S, d->dinv, d->pA, d->pB, d->q, d->x,
+                                                     d->r, d->z, d->part, d->st);
+    }
+    MM_HIP(hipGetLastError());
+    rc = get_state(d, &h);
+    if (rc) return rc;
+  }
+  MM_HIP(hipMemcpyAsync(x, d->x, d->nrow * sizeof(double), hipMemcpyDeviceToHost, d->stream));
+  MM_HIP(hipStreamSynchronize(d->stream));
+  const double relres = (h.bnorm == 0.0) ? 0.0 : h.rnorm / h.bnorm;
+  if (solver) {
+    solver->its = h.its;
+    solver->err = relres;
+  }
+  return (h.bnorm == 0.0 || h.rnorm <= rtol * h.bnorm) ? 0 : MM_NOT_CONVERGED;
+}
+
+extern "C" int ell_device_destroy(ell_device *d)
+{
+  if (!d) return 1;
+  if (d->stream) hipStreamSynchronize(d->stream);
+  hipFree(d->cols);
+  hipFree(d->vals);
+  hipFree(d->dinv);
+  hipFree(d->x);
+  hipFree(d->b);
+  hipFree(d->r);
+  hipFree(d->z);
+  hipFree(d->pA);
+  hipFree(d->pB);
+  hipFree(d->q);
+  hipFree(d->part);
+  hipFree(d->st);
+  if (d->stream) hipStreamDestroy(d->stream);
+  free(d);
+  return 0;
+}
+
+// ---- reference entry points running on the GPU ------------------------------
+
+// src/ell.c:127-141
+extern "C" int ell_mvp(ell_matrix *m, double *x, double *y)
+{
+  if (m == NULL || x == NULL || y == NULL) return 1;
+  ell_device *d = NULL;
+  int rc = ell_device_create(m, 0, &d);
+  if (rc) return rc;
+  rc = ell_device_mvp(d, x, y);
+  ell_device_destroy(d);
+  return rc;
+}
+
+// src/ell.c:143-184
+extern "C" int ell_solve_jacobi(ell_solver *solver, ell_matrix *m, double *b, double *x)
+{
+  if (m == NULL || b == NULL || x == NULL) return 1;
+  ell_device *d = NULL;
+  int rc = ell_device_create(m, 0, &d);
+  if (rc) return rc;
+  rc = ell_device_solve_jacobi(d, solver, b, x);
+  ell_device_destroy(d);
+  return rc;
+}
+
+extern "C" int ell_solve_pcg(ell_solver *solver, ell_matrix *m, double *b, double *x)
+{
+  if (m == NULL || b == NULL || x == NULL) return 1;
+  ell_device *d = NULL;
+  int rc = ell_device_create(m, 0, &d);
+  if (rc) return rc;
+  rc = ell_device_solve_pcg(d, solver, b, x);
+  ell_device_destroy(d);
+  return rc;
+}

@@ -1,0 +1,949 @@
+// engine.hip -- host side of the RVE engine: HBM arena, admission by
+// footprint, resident-RVE cache, hipGraph-captured solver steps, retirement,
+// and the serving worker thread.  C-ABI in include/micro_api.h.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <cmath>
+#include <condition_variable>
+#include <cstdlib>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <mutex>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+#include "micro_api.h"
+#include "mm_internal.h"
+
+using namespace mm;
+
+namespace {
+
+// First-fit allocator of index ranges [0, cap).
+struct RangeAlloc {
+  std::map<long long, long long> free_;  // start -> length
+  long long cap = 0;
+  void init(long long c)
+  {
+    free_.clear();
+    cap = c;
+    if (c > 0) free_[0] = c;
+  }
+  long long alloc(long long n)
+  {
+    for (auto it = free_.begin(); it != free_.end(); ++it) {
+      if (it->second >= n) {
+        const long long s = it->first, len = it->second;
+        free_.erase(it);
+        if (len > n) free_[s + n] = len - n;
+        return s;
+      }
+    }
+    return -1;
+  }
+  void release(long long s, long long n)
+  {
+    auto it = free_.emplace(s, n).first;
+    auto nx = std::next(it);
+    if (nx != free_.end() && it->first + it->second == nx->first) {
+      it->second += nx->second;
+      free_.erase(nx);
+    }
+    if (it != free_.begin()) {
+      auto pv = std::prev(it);
+      if (pv->first + pv->second == it->first) {
+        pv->second += it->second;
+        free_.erase(it);
+      }
+    }
+  }
+  long long high_water() const
+  {
+    if (free_.empty()) return cap;
+    auto last = std::prev(free_.end());
+    return (last->first + last->second == cap) ? last->first : cap;
+  }
+};
+
+enum SlotState { SL_FREE = 0, SL_ACTIVE = 1, SL_CACHED = 2 };
+
+struct HostSlot {
+  int state = SL_FREE;
+  mm_request req;
+  long long first_slab = 0, nslab = 0, elem_base = 0, nelm = 0;
+  int nrow = 0;
+  double t_submit = 0, t_admit = 0;
+  uint64_t lru = 0;
+};
+
+struct Queued {
+  mm_request req;
+  double t_submit;
+};
+
+bool same_geometry(const mm_rve_spec &a, const mm_rve_spec &b)
+{
+  if (a.nex != b.nex || a.ney != b.ney || a.nez != b.nez || a.lx != b.lx || a.ly != b.ly ||
+      a.lz != b.lz || a.nfib != b.nfib)
+    return false;
+  for (int i = 0; i < a.nfib; i++)
+    if (a.fib_x[i] != b.fib_x[i] || a.fib_y[i] != b.fib_y[i] || a.fib_r[i] != b.fib_r[i]) return false;
+  for (int i = 0; i < 2; i++)
+    if (a.young[i] != b.young[i] || a.poisson[i] != b.poisson[i]) return false;
+  return true;
+}
+
+int spec_valid(const mm_rve_spec &s)
+{
+  if (s.nex < 1 || s.ney < 1 || s.nez < 1) return 0;
+  if (!(s.lx > 0) || !(s.ly > 0) || !(s.lz > 0)) return 0;
+  if (s.nfib < 0 || s.nfib > MM_MAX_FIBERS) return 0;
+  const double nn = (double)(s.nex + 1) * (s.ney + 1) * (s.nez + 1);
+  if (nn * 3 > 2.0e9) return 0;
+  for (int i = 0; i < 2; i++) {
+    if (!std::isfinite(s.young[i]) || !(s.young[i] > 0)) return 0;
+    if (!(s.poisson[i] > -1.0 && s.poisson[i] < 0.5)) return 0;
+  }
+  for (int i = 0; i < s.nfib; i++)
+    if (!std::isfinite(s.fib_x[i]) || !std::isfinite(s.fib_y[i]) || !std::isfinite(s.fib_r[i]))
+      return 0;
+  return 1;
+}
+
+template <typename T>
+int ensure_pinned(T **p, size_t *cap, size_t n)
+{
+  if (n <= *cap) return 0;
+  if (*p) hipHostFree(*p);
+  *p = nullptr;
+  size_t c = std::max<size_t>(n, 2 * *cap);
+  if (hipHostMalloc((void **)p, c * sizeof(T), hipHostMallocDefault) != hipSuccess) {
+    *cap = 0;
+    return MM_ERR_HIP;
+  }
+  *cap = c;
+  return 0;
+}
+
+template <typename T>
+int ensure_device(T **p, size_t *cap, size_t n)
+{
+  if (n <= *cap) return 0;
+  if (*p) hipFree(*p);
+  *p = nullptr;
+  size_t c = std::max<size_t>(n, 2 * *cap);
+  if (hipMalloc((void **)p, c * sizeof(T)) != hipSuccess) {
+    *cap = 0;
+    return MM_ERR_HIP;
+  }
+  *cap = c;
+  return 0;
+}
+
+}  // namespace
+
+struct mm_engine {
+  mm_engine_config cfg;
+  int S = 256, K = 1, KS = 1;
+  bool fp64 = false;
+  size_t tsz = 4;
+  hipStream_t stream = nullptr;
+  // arena
+  long long nslab_cap = 0, elem_cap = 0;
+  int rve_cap = 0;
+  int *cols = nullptr;
+  void *vals = nullptr, *dinv = nullptr;
+  double *x64 = nullptr;
+  void *d = nullptr, *r = nullptr, *z = nullptr, *pA = nullptr, *pB = nullptr, *q = nullptr;
+  double *part = nullptr;
+  int *slab_rve = nullptr, *slab_row0 = nullptr;
+  RveDev *rves = nullptr;
+  uint8_t *elem_mat = nullptr;
+  double *ke_tab = nullptr, *c_tab = nullptr, *bw_tab = nullptr;
+  RveOut *out_host = nullptr, *out_dev = nullptr;
+  size_t arena_bytes = 0;
+  // staging
+  RveDev *st_states = nullptr;
+  int *st_slots = nullptr, *st_ke = nullptr, *st_slabs = nullptr;
+  int2 *st_phase = nullptr;
+  size_t cap_states = 0, cap_slots = 0, cap_ke = 0, cap_slabs = 0, cap_phase = 0;
+  RveDev *dv_states = nullptr;
+  int *dv_slots = nullptr, *dv_ke = nullptr, *dv_slabs = nullptr;
+  int2 *dv_phase = nullptr;
+  size_t dcap_states = 0, dcap_slots = 0, dcap_ke = 0, dcap_slabs = 0, dcap_phase = 0;
+  // bookkeeping
+  RangeAlloc slabs, elems;
+  std::vector<HostSlot> slots;
+  std::vector<int> free_slots;
+  std::unordered_map<uint64_t, int> warm_map;
+  uint64_t lru_clock = 0;
+  int n_active = 0;
+  // queue / results
+  std::mutex qmu;
+  std::condition_variable qcv;
+  std::deque<Queued> queue;
+  std::deque<mm_result> results;
+  // graph
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  int graph_grid = -1;
+  // stats / threading
+  std::mutex step_mu;
+  long long completed = 0, nsteps = 0;
+  double spmv_ms = 0.0;
+  std::thread worker;
+  std::atomic<bool> running{false};
+  std::chrono::steady_clock::time_point t0;
+  int last_error = 0;
+
+  EngPtrs ptrs() const
+  {
+    EngPtrs P;
+    P.cols = cols;
+    P.vals = vals;
+    P.dinv = dinv;
+    P.x64 = x64;
+    P.d = d;
+    P.r = r;
+    P.z = z;
+    P.pA = pA;
+    P.pB = pB;
+    P.q = q;
+    P.part = part;
+    P.slab_rve = slab_rve;
+    P.slab_row0 = slab_row0;
+    P.rves = rves;
+    P.out = out_dev;
+    P.ke_tab = ke_tab;
+    P.c_tab = c_tab;
+    P.bw_tab = bw_tab;
+    P.elem_mat = elem_mat;
+    P.S = S;
+    return P;
+  }
+  SolveParams params() const
+  {
+    SolveParams sp;
+    sp.nrhs = K;
+    sp.slab_rows = S;
+    sp.rtol = cfg.rtol;
+    sp.abs_tol = cfg.abs_tol;
+    sp.inner_rtol = cfg.inner_rtol;
+    sp.max_inner_its = cfg.max_inner_its;
+    sp.max_sweeps = cfg.max_sweeps;
+    return sp;
+  }
+  double now() const
+  {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  }
+  size_t bytes_per_slab() const
+  {
+    return (size_t)S * kW * (tsz + 4) + (size_t)S * (KS * (8 + 6 * tsz) + tsz) +
+           kPartDoubles * 8 + 8 + (size_t)S / 3 + 1;
+  }
+};
+
+static int engine_free_arena(mm_engine *e)
+{
+  void *ptrs[] = {e->cols, e->vals, e->dinv, e->x64, e->d, e->r, e->z, e->pA, e->pB, e->q, e->part,
+                  e->slab_rve, e->slab_row0, e->rves, e->elem_mat, e->ke_tab, e->c_tab, e->bw_tab,
+                  e->dv_states, e->dv_slots, e->dv_ke, e->dv_slabs, e->dv_phase};
+  for (void *p : ptrs)
+    if (p) hipFree(p);
+  void *hp[] = {e->out_host, e->st_states, e->st_slots, e->st_ke, e->st_slabs, e->st_phase};
+  for (void *p : hp)
+    if (p) hipHostFree(p);
+  return 0;
+}
+
+extern "C" int mm_engine_config_default(mm_engine_config *c)
+{
+  if (!c) return MM_ERR_ARG;
+  memset(c, 0, sizeof(*c));
+  c->device = 0;
+  c->hbm_budget = 0;
+  c->max_rves = 0;
+  c->slab_rows = 0;
+  c->nrhs = 6;
+  c->precision = MM_PREC_MIXED;
+  c->rtol = 1.0e-10;
+  c->abs_tol = 0.0;
+  c->inner_rtol = 1.0e-5;
+  c->max_inner_its = 20000;
+  c->max_sweeps = 8;
+  c->steps_per_launch = 16;
+  c->use_graph = 1;
+  c->sync_every_step = 0;
+  return 0;
+}
+
+extern "C" int mm_engine_create(const mm_engine_config *cfg, mm_engine **out)
+{
+  if (!cfg || !out) return MM_ERR_ARG;
+  if (!eng_supported_rhs(cfg->nrhs)) {
+    fprintf(stderr, "madmax_amd: nrhs %d not supported (1, 6 or 7)\n", cfg->nrhs);
+    return MM_ERR_ARG;
+  }
+  if (cfg->slab_rows != 0 && (cfg->slab_rows % 64 != 0 || cfg->slab_rows > 512)) return MM_ERR_ARG;
+  if (cfg->precision != MM_PREC_FP64 && cfg->precision != MM_PREC_MIXED) return MM_ERR_ARG;
+  if (!(cfg->rtol >= 0) || !(cfg->inner_rtol > 0) || cfg->max_inner_its < 1 || cfg->max_sweeps < 1)
+    return MM_ERR_ARG;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) {
+    fprintf(stderr, "madmax_amd: no HIP device available\n");
+    return MM_ERR_HIP;
+  }
+  if (cfg->device < 0 || cfg->device >= ndev) return MM_ERR_ARG;
+  MM_HIP(hipSetDevice(cfg->device));
+  mm_engine *e = new mm_engine();
+  e->cfg = *cfg;
+  if (e->cfg.steps_per_launch < 1) e->cfg.steps_per_launch = 1;
+  if (e->cfg.sync_every_step) {
+    e->cfg.steps_per_launch = 1;
+    e->cfg.use_graph = 0;
+  }
+  e->S = cfg->slab_rows ? cfg->slab_rows : 256;
+  e->K = cfg->nrhs;
+  e->KS = vec_stride_host(e->K);
+  e->fp64 = (cfg->precision == MM_PREC_FP64);
+  e->tsz = e->fp64 ? 8 : 4;
+  e->rve_cap = cfg->max_rves > 0 ? cfg->max_rves : 4096;
+  e->t0 = std::chrono::steady_clock::now();
+  int rc = 0;
+  size_t budget = cfg->hbm_budget;
+  if (budget == 0) {
+    size_t fr = 0, tot = 0;
+    if (hipMemGetInfo(&fr, &tot) != hipSuccess) {
+      delete e;
+      return MM_ERR_HIP;
+    }
+    budget = (size_t)(0.8 * (double)fr);
+  }
+  const size_t per_rve = (size_t)kNmat * (kKe + 36) * 8 + 144 * 8 + sizeof(RveDev) + sizeof(RveOut);
+  if (budget <= (size_t)e->rve_cap * per_rve + e->bytes_per_slab()) {
+    delete e;
+    return MM_ERR_ALLOC;
+  }
+  e->nslab_cap = (long long)((budget - (size_t)e->rve_cap * per_rve) / e->bytes_per_slab());
+  e->elem_cap = e->nslab_cap * e->S / 3 + 64;
+  const size_t nrows = (size_t)e->nslab_cap * e->S;
+  const size_t slots = nrows * kW;
+  if (hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete e;
+    return MM_ERR_HIP;
+  }
+#define EALLOC(ptr, bytes)                                                  \
+  do {                                                                      \
+    if (hipMalloc((void **)&(ptr), (bytes)) != hipSuccess) {                \
+      fprintf(stderr, "madmax_amd: arena allocation of %zu bytes failed\n", \
+              (size_t)(bytes));                                             \
+      rc = MM_ERR_ALLOC;                                                    \
+      goto fail;                                                            \
+    }                                                                       \
+    e->arena_bytes += (bytes);                                              \
+  } while (0)
+  EALLOC(e->cols, slots * 4);
+  EALLOC(e->vals, slots * e->tsz);
+  EALLOC(e->dinv, nrows * e->tsz);
+  EALLOC(e->x64, nrows * e->KS * 8);
+  EALLOC(e->d, nrows * e->KS * e->tsz);
+  EALLOC(e->r, nrows * e->KS * e->tsz);
+  EALLOC(e->z, nrows * e->KS * e->tsz);
+  EALLOC(e->pA, nrows * e->KS * e->tsz);
+  EALLOC(e->pB, nrows * e->KS * e->tsz);
+  EALLOC(e->q, nrows * e->KS * e->tsz);
+  EALLOC(e->part, (size_t)e->nslab_cap * kPartDoubles * 8);
+  EALLOC(e->slab_rve, (size_t)e->nslab_cap * 4);
+  EALLOC(e->slab_row0, (size_t)e->nslab_cap * 4);
+  EALLOC(e->rves, (size_t)e->rve_cap * sizeof(RveDev));
+  EALLOC(e->elem_mat, (size_t)e->elem_cap);
+  EALLOC(e->ke_tab, (size_t)e->rve_cap * kNmat * kKe * 8);
+  EALLOC(e->c_tab, (size_t)e->rve_cap * kNmat * 36 * 8);
+  EALLOC(e->bw_tab, (size_t)e->rve_cap * 144 * 8);
+#undef EALLOC
+  if (hipHostMalloc((void **)&e->out_host, (size_t)e->rve_cap * sizeof(RveOut), hipHostMallocMapped) !=
+      hipSuccess) {
+    rc = MM_ERR_HIP;
+    goto fail;
+  }
+  memset(e->out_host, 0, (size_t)e->rve_cap * sizeof(RveOut));
+  if (hipHostGetDevicePointer((void **)&e->out_dev, e->out_host, 0) != hipSuccess) {
+    rc = MM_ERR_HIP;
+    goto fail;
+  }
+  if (hipMemsetAsync(e->slab_rve, 0xff, (size_t)e->nslab_cap * 4, e->stream) != hipSuccess ||
+      hipMemsetAsync(e->rves, 0, (size_t)e->rve_cap * sizeof(RveDev), e->stream) != hipSuccess) {
+    rc = MM_ERR_HIP;
+    goto fail;
+  }
+  rc = upload_slot_tables(e->stream);
+  if (rc) goto fail;
+  e->slabs.init(e->nslab_cap);
+  e->elems.init(e->elem_cap);
+  e->slots.resize(e->rve_cap);
+  for (int i = e->rve_cap - 1; i >= 0; i--) e->free_slots.push_back(i);
+  *out = e;
+  return 0;
+fail:
+  engine_free_arena(e);
+  if (e->stream) hipStreamDestroy(e->stream);
+  delete e;
+  return rc;
+}
+
+extern "C" int mm_engine_stop(mm_engine *e);
+
+extern "C" int mm_engine_destroy(mm_engine *e)
+{
+  if (!e) return MM_ERR_ARG;
+  mm_engine_stop(e);
+  if (e->stream) hipStreamSynchronize(e->stream);
+  if (e->gexec) hipGraphExecDestroy(e->gexec);
+  if (e->graph) hipGraphDestroy(e->graph);
+  engine_free_arena(e);
+  if (e->stream) hipStreamDestroy(e->stream);
+  delete e;
+  return 0;
+}
+
+static void footprint(const mm_engine *e, const mm_rve_spec &s, int *nrow, long long *nslab,
+                      long long *nelm)
+{
+  const long long nn = (long long)(s.nex + 1) * (s.ney + 1) * (s.nez + 1);
+  *nrow = (int)(3 * nn);
+  *nslab = (3 * nn + e->S - 1) / e->S;
+  *nelm = (long long)s.nex * s.ney * s.nez;
+}
+
+extern "C" int mm_engine_submit(mm_engine *e, const mm_request *req)
+{
+  if (!e || !req) return MM_ERR_ARG;
+  if (!spec_valid(req->rve) || req->nload < 1 || req->nload > e->K) return MM_ERR_ARG;
+  if (req->tangent_mode == MM_TANGENT_UNIT && req->nload < 6) return MM_ERR_ARG;
+  if (req->tangent_mode == MM_TANGENT_FD && req->nload < 7) return MM_ERR_ARG;
+  for (int k = 0; k < req->nload; k++)
+    for (int i = 0; i < MM_NVOI; i++)
+      if (!std::isfinite(req->strain[k][i])) return MM_ERR_ARG;
+  if (req->tangent_mode == MM_TANGENT_UNIT)
+    for (int i = 0; i < 6; i++)
+      if (req->strain[i][i] == 0.0) return MM_ERR_ARG;
+  if (req->tangent_mode == MM_TANGENT_FD)
+    for (int i = 0; i < 6; i++)
+      if (req->strain[i + 1][i] == req->strain[0][i]) return MM_ERR_ARG;
+  int nrow;
+  long long nslab, nelm;
+  footprint(e, req->rve, &nrow, &nslab, &nelm);
+  if (nslab > e->nslab_cap || nelm > e->elem_cap) return MM_REJECTED;
+  {
+    std::lock_guard<std::mutex> lk(e->qmu);
+    e->queue.push_back(Queued{*req, e->now()});
+  }
+  e->qcv.notify_all();
+  return 0;
+}
+
+static void release_slot_resources(mm_engine *e, int slot)
+{
+  HostSlot &h = e->slots[slot];
+  e->slabs.release(h.first_slab, h.nslab);
+  e->elems.release(h.elem_base, h.nelm);
+  hipMemsetAsync(e->slab_rve + h.first_slab, 0xff, (size_t)h.nslab * 4, e->stream);
+  if (h.req.warm_key) {
+    auto it = e->warm_map.find(h.req.warm_key);
+    if (it != e->warm_map.end() && it->second == slot) e->warm_map.erase(it);
+  }
+  h.state = SL_FREE;
+  e->free_slots.push_back(slot);
+}
+
+static bool evict_one(mm_engine *e)
+{
+  int best = -1;
+  for (int i = 0; i < e->rve_cap; i++)
+    if (e->slots[i].state == SL_CACHED && (best < 0 || e->slots[i].lru < e->slots[best].lru)) best = i;
+  if (best < 0) return false;
+  release_slot_resources(e, best);
+  return true;
+}
+
+static void fill_state(const mm_engine *e, const HostSlot &h, RveDev *st, int warm)
+{
+  memset(st, 0, sizeof(*st));
+  const mm_rve_spec &s = h.req.rve;
+  st->nex = s.nex;
+  st->ney = s.ney;
+  st->nez = s.nez;
+  st->nx = s.nex + 1;
+  st->ny = s.ney + 1;
+  st->nz = s.nez + 1;
+  st->nrow = h.nrow;
+  st->nelm = (int)h.nelm;
+  st->nslab = (int)h.nslab;
+  st->first_slab = (int)h.first_slab;
+  st->row_base = h.first_slab * e->S;
+  st->elem_base = h.elem_base;
+  st->hx = s.lx / s.nex;
+  st->hy = s.ly / s.ney;
+  st->hz = s.lz / s.nez;
+  st->vol = s.lx * s.ly * s.lz;
+  st->nfib = s.nfib;
+  for (int i = 0; i < s.nfib; i++) {
+    st->fx[i] = s.fib_x[i];
+    st->fy[i] = s.fib_y[i];
+    st->fr[i] = s.fib_r[i];
+  }
+  for (int i = 0; i < kNmat; i++) {
+    st->young[i] = s.young[i];
+    st->poisson[i] = s.poisson[i];
+  }
+  st->tangent_mode = h.req.tangent_mode;
+  st->nload = h.req.nload;
+  st->warm = warm;
+  for (int k = 0; k < h.req.nload; k++)
+    for (int i = 0; i < MM_NVOI; i++) st->load[k][i] = h.req.strain[k][i];
+  st->phase = PH_RES;
+}
+
+// Admits queued requests that fit; uploads their state and assembles them.
+static int admit(mm_engine *e)
+{
+  std::vector<int> fresh, reused;
+  {
+    std::lock_guard<std::mutex> lk(e->qmu);
+    while (!e->queue.empty()) {
+      Queued &qd = e->queue.front();
+      const mm_request &req = qd.req;
+      // a resident RVE cached under the same key and geometry is reused
+      if (req.warm_key) {
+        auto it = e->warm_map.find(req.warm_key);
+        if (it != e->warm_map.end()) {
+          HostSlot &h = e->slots[it->second];
+          if (h.state == SL_ACTIVE) break;  // same key still in flight: keep FIFO order
+          if (h.state == SL_CACHED && same_geometry(h.req.rve, req.rve)) {
+            h.req = req;
+            h.state = SL_ACTIVE;
+            h.t_submit = qd.t_submit;
+            h.t_admit = e->now();
+            reused.push_back(it->second);
+            e->queue.pop_front();
+            continue;
+          }
+          release_slot_resources(e, it->second);  // stale geometry under this key
+        }
+      }
+      int nrow;
+      long long nslab, nelm;
+      footprint(e, req.rve, &nrow, &nslab, &nelm);
+      long long fs = -1, fe = -1;
+      int slot = -1;
+      for (;;) {
+        if (fs < 0) fs = e->slabs.alloc(nslab);
+        if (fe < 0) fe = e->elems.alloc(nelm);
+        if (slot < 0 && !e->free_slots.empty()) {
+          slot = e->free_slots.back();
+          e->free_slots.pop_back();
+        }
+        if (fs >= 0 && fe >= 0 && slot >= 0) break;
+        if (!evict_one(e)) break;
+      }
+      if (fs < 0 || fe < 0 || slot < 0) {
+        if (fs >= 0) e->slabs.release(fs, nslab);
+        if (fe >= 0) e->elems.release(fe, nelm);
+        if (slot >= 0) e->free_slots.push_back(slot);
+        break;  // wait for resident RVEs to finish
+      }
+      HostSlot &h = e->slots[slot];
+      h.state = SL_ACTIVE;
+      h.req = req;
+      h.first_slab = fs;
+      h.nslab = nslab;
+      h.elem_base = fe;
+      h.nelm = nelm;
+      h.nrow = nrow;
+      h.t_submit = qd.t_submit;
+      h.t_admit = e->now();
+      if (req.warm_key) e->warm_map[req.warm_key] = slot;
+      fresh.push_back(slot);
+      e->queue.pop_front();
+    }
+  }
+  const size_t n = fresh.size() + reused.size();
+  if (n == 0) return 0;
+  int rc = 0;
+  // staging: states, slot ids, Ke work, phase-map blocks, slab lists
+  size_t nke = fresh.size() * kNmat, nph = 0, nsl = 0, nsl_re = 0;
+  for (int s : fresh) {
+    nph += (e->slots[s].nelm + 255) / 256;
+    nsl += e->slots[s].nslab;
+  }
+  for (int s : reused) nsl_re += e->slots[s].nslab;
+  if ((rc = ensure_pinned(&e->st_states, &e->cap_states, n))) return rc;
+  if ((rc = ensure_pinned(&e->st_slots, &e->cap_slots, n))) return rc;
+  if ((rc = ensure_pinned(&e->st_ke, &e->cap_ke, 2 * nke + 2))) return rc;
+  if ((rc = ensure_pinned(&e->st_phase, &e->cap_phase, nph + 1))) return rc;
+  if ((rc = ensure_pinned(&e->st_slabs, &e->cap_slabs, nsl + nsl_re + 1))) return rc;
+  if ((rc = ensure_device(&e->dv_states, &e->dcap_states, n))) return rc;
+  if ((rc = ensure_device(&e->dv_slots, &e->dcap_slots, n))) return rc;
+  if ((rc = ensure_device(&e->dv_ke, &e->dcap_ke, 2 * nke + 2))) return rc;
+  if ((rc = ensure_device(&e->dv_phase, &e->dcap_phase, nph + 1))) return rc;
+  if ((rc = ensure_device(&e->dv_slabs, &e->dcap_slabs, nsl + nsl_re + 1))) return rc;
+  size_t i = 0, ik = 0, ip = 0, is = 0;
+  for (int s : fresh) {
+    const HostSlot &h = e->slots[s];
+    fill_state(e, h, &e->st_states[i], 0);
+    e->st_slots[i++] = s;
+    for (int m = 0; m < kNmat; m++) {
+      e->st_ke[2 * ik] = s;
+      e->st_ke[2 * ik + 1] = m;
+      ik++;
+    }
+    for (long long e0 = 0; e0 < h.nelm; e0 += 256) e->st_phase[ip++] = make_int2(s, (int)e0);
+    for (long long b = 0; b < h.nslab; b++) e->st_slabs[is++] = (int)(h.first_slab + b);
+  }
+  for (int s : reused) {
+    const HostSlot &h = e->slots[s];
+    fill_state(e, h, &e->st_states[i], h.req.warm_start ? 1 : 0);
+    e->st_slots[i++] = s;
+    for (long long b = 0; b < h.nslab; b++) e->st_slabs[is++] = (int)(h.first_slab + b);
+  }
+  for (int s : fresh) e->out_host[s].phase = 0;
+  for (int s : reused) e->out_host[s].phase = 0;
+  MM_HIP(hipMemcpyAsync(e->dv_states, e->st_states, n * sizeof(RveDev), hipMemcpyHostToDevice, e->stream));
+  MM_HIP(hipMemcpyAsync(e->dv_slots, e->st_slots, n * sizeof(int), hipMemcpyHostToDevice, e->stream));
+  if (nke)
+    MM_HIP(hipMemcpyAsync(e->dv_ke, e->st_ke, 2 * nke * sizeof(int), hipMemcpyHostToDevice, e->stream));
+  if (nph)
+    MM_HIP(hipMemcpyAsync(e->dv_phase, e->st_phase, nph * sizeof(int2), hipMemcpyHostToDevice, e->stream));
+  if (nsl + nsl_re)
+    MM_HIP(hipMemcpyAsync(e->dv_slabs, e->st_slabs, (nsl + nsl_re) * sizeof(int), hipMemcpyHostToDevice,
+                          e->stream));
+  EngPtrs P = e->ptrs();
+  if ((rc = eng_launch_admit((int)n, e->dv_slots, e->dv_states, e->rves, e->slab_rve, e->slab_row0, e->S,
+                             e->stream)))
+    return rc;
+  if ((rc = eng_launch_ke((int)nke, e->dv_ke, e->rves, e->ke_tab, e->c_tab, e->bw_tab, e->stream)))
+    return rc;
+  if ((rc = eng_launch_phase((int)nph, e->dv_phase, e->rves, e->elem_mat, e->stream))) return rc;
+  if ((rc = eng_launch_assemble(e->fp64, e->K, P, (int)nsl, e->dv_slabs, 0, e->stream))) return rc;
+  if ((rc = eng_launch_assemble(e->fp64, e->K, P, (int)nsl_re, e->dv_slabs + nsl, 1, e->stream)))
+    return rc;
+  e->n_active += (int)n;
+  return 0;
+}
+
+static int run_steps(mm_engine *e)
+{
+  long long hw = e->slabs.high_water();
+  if (hw <= 0) return 0;
+  int grid = (int)(((hw + 63) / 64) * 64);
+  if (grid > e->nslab_cap) grid = (int)e->nslab_cap;
+  EngPtrs P = e->ptrs();
+  SolveParams sp = e->params();
+  const int T = e->cfg.steps_per_launch;
+  if (e->cfg.use_graph) {
+    if (!e->gexec || e->graph_grid != grid) {
+      if (e->gexec) hipGraphExecDestroy(e->gexec);
+      if (e->graph) hipGraphDestroy(e->graph);
+      e->gexec = nullptr;
+      e->graph = nullptr;
+      MM_HIP(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
+      int rc = eng_launch_steps(e->fp64, e->K, P, sp, grid, T, e->stream);
+      hipGraph_t g = nullptr;
+      hipError_t ce = hipStreamEndCapture(e->stream, &g);
+      if (rc) return rc;
+      MM_HIP(ce);
+      e->graph = g;
+      MM_HIP(hipGraphInstantiate(&e->gexec, g, nullptr, nullptr, 0));
+      e->graph_grid = grid;
+    }
+    MM_HIP(hipGraphLaunch(e->gexec, e->stream));
+  } else {
+    int rc = eng_launch_steps(e->fp64, e->K, P, sp, grid, T, e->stream);
+    if (rc) return rc;
+  }
+  return 0;
+}
+
+static void retire(mm_engine *e)
+{
+  for (int s = 0; s < e->rve_cap; s++) {
+    HostSlot &h = e->slots[s];
+    if (h.state != SL_ACTIVE) continue;
+    volatile RveOut *o = e->out_host + s;
+    if (o->phase != PH_DONE) continue;
+    mm_result res;
+    memset(&res, 0, sizeof(res));
+    res.id = h.req.id;
+    res.status = o->status;
+    res.pcg_its = o->its_total;
+    res.sweeps = o->sweeps;
+    res.residual_rel = o->residual_rel;
+    for (int k = 0; k < MM_MAX_RHS; k++)
+      for (int i = 0; i < MM_NVOI; i++) {
+        res.stress[k][i] = o->stress[k][i];
+        res.strain[k][i] = o->strain[k][i];
+      }
+    for (int i = 0; i < 36; i++) res.c_tangent[i] = o->c[i];
+    res.t_submit = h.t_submit;
+    res.t_admit = h.t_admit;
+    res.t_done = e->now();
+    o->phase = 0;
+    e->n_active--;
+    e->completed++;
+    if (h.req.warm_key) {
+      h.state = SL_CACHED;
+      h.lru = ++e->lru_clock;
+    } else {
+      release_slot_resources(e, s);
+    }
+    std::lock_guard<std::mutex> lk(e->qmu);
+    e->results.push_back(res);
+  }
+}
+
+extern "C" int mm_engine_step(mm_engine *e)
+{
+  if (!e) return MM_ERR_ARG;
+  std::lock_guard<std::mutex> lk(e->step_mu);
+  MM_HIP(hipSetDevice(e->cfg.device));
+  int rc = admit(e);
+  if (rc) return e->last_error = rc;
+  if (e->n_active > 0) {
+    rc = run_steps(e);
+    if (rc) return e->last_error = rc;
+  }
+  MM_HIP(hipStreamSynchronize(e->stream));
+  e->nsteps++;
+  retire(e);
+  return 0;
+}
+
+extern "C" int mm_engine_poll(mm_engine *e, mm_result *out, int max, int *n)
+{
+  if (!e || !n || (max > 0 && !out)) return MM_ERR_ARG;
+  std::lock_guard<std::mutex> lk(e->qmu);
+  int c = 0;
+  while (c < max && !e->results.empty()) {
+    out[c++] = e->results.front();
+    e->results.pop_front();
+  }
+  *n = c;
+  return 0;
+}
+
+static bool has_work(mm_engine *e)
+{
+  std::lock_guard<std::mutex> lk(e->qmu);
+  return !e->queue.empty() || e->n_active > 0;
+}
+
+extern "C" int mm_engine_drain(mm_engine *e)
+{
+  if (!e) return MM_ERR_ARG;
+  long long stall = 0;
+  size_t last_q = (size_t)-1;
+  // every RVE finishes within max_sweeps * (max_inner_its + 3) PCG steps;
+  // a drain that makes no progress for longer than that reports an error
+  const long long limit =
+      (long long)e->cfg.max_sweeps * (e->cfg.max_inner_its + 4) / e->cfg.steps_per_launch + 64;
+  long long idle = 0, done0 = e->completed;
+  while (has_work(e)) {
+    int rc = mm_engine_step(e);
+    if (rc) return rc;
+    if (e->completed != done0) {
+      done0 = e->completed;
+      idle = 0;
+    } else if (++idle > limit) {
+      fprintf(stderr, "madmax_amd: drain made no progress in %lld steps\n", idle);
+      return MM_ERR_STATE;
+    }
+    size_t qn;
+    int na;
+    {
+      std::lock_guard<std::mutex> lk(e->qmu);
+      qn = e->queue.size();
+      na = e->n_active;
+    }
+    // nothing resident and the head of the queue still does not fit
+    if (na == 0 && qn > 0 && qn == last_q) {
+      if (++stall > 2) return MM_ERR_FULL;
+    } else {
+      stall = 0;
+    }
+    last_q = qn;
+  }
+  return 0;
+}
+
+static void worker_main(mm_engine *e)
+{
+  hipSetDevice(e->cfg.device);
+  while (e->running.load()) {
+    if (!has_work(e)) {
+      std::unique_lock<std::mutex> lk(e->qmu);
+      e->qcv.wait_for(lk, std::chrono::microseconds(200),
+                      [&] { return !e->queue.empty() || !e->running.load(); });
+      continue;
+    }
+    if (mm_engine_step(e)) {
+      e->running.store(false);
+      break;
+    }
+  }
+}
+
+extern "C" int mm_engine_start(mm_engine *e)
+{
+  if (!e) return MM_ERR_ARG;
+  if (e->running.load()) return 0;
+  e->running.store(true);
+  e->worker = std::thread(worker_main, e);
+  return 0;
+}
+
+extern "C" int mm_engine_stop(mm_engine *e)
+{
+  if (!e) return MM_ERR_ARG;
+  if (e->running.exchange(false)) e->qcv.notify_all();
+  if (e->worker.joinable()) e->worker.join();
+  return e->last_error;
+}
+
+extern "C" double mm_engine_now(const mm_engine *e) { return e ? e->now() : 0.0; }
+
+extern "C" int mm_engine_info_get(mm_engine *e, mm_engine_info *info)
+{
+  if (!e || !info) return MM_ERR_ARG;
+  std::lock_guard<std::mutex> lk(e->qmu);
+  info->resident = e->n_active;
+  info->queued = (int)e->queue.size();
+  info->completed = e->completed;
+  info->steps = e->nsteps;
+  info->arena_bytes = e->arena_bytes;
+  long long free_slabs = 0;
+  for (auto &kv : e->slabs.free_) free_slabs += kv.second;
+  info->arena_used = (e->slabs.cap > 0)
+                         ? (size_t)((double)e->arena_bytes * (double)(e->slabs.cap - free_slabs) /
+                                    (double)e->slabs.cap)
+                         : 0;
+  info->slab_rows = e->S;
+  info->spmv_ms = e->spmv_ms;
+  return 0;
+}
+
+static int find_cached(mm_engine *e, uint64_t key)
+{
+  auto it = e->warm_map.find(key);
+  if (it == e->warm_map.end()) return -1;
+  if (e->slots[it->second].state != SL_CACHED) return -1;
+  return it->second;
+}
+
+extern "C" int mm_engine_rve_sizes(mm_engine *e, uint64_t key, int *nrow, int *nelm)
+{
+  if (!e) return MM_ERR_ARG;
+  std::lock_guard<std::mutex> lk(e->step_mu);
+  const int s = find_cached(e, key);
+  if (s < 0) return MM_ERR_STATE;
+  if (nrow) *nrow = e->slots[s].nrow;
+  if (nelm) *nelm = (int)e->slots[s].nelm;
+  return 0;
+}
+
+extern "C" int mm_engine_download_ell(mm_engine *e, uint64_t key, int *cols, double *vals)
+{
+  if (!e || !cols || !vals) return MM_ERR_ARG;
+  std::lock_guard<std::mutex> lk(e->step_mu);
+  const int s = find_cached(e, key);
+  if (s < 0) return MM_ERR_STATE;
+  const HostSlot &h = e->slots[s];
+  const size_t n = (size_t)h.nslab * e->S * kW;
+  std::vector<int> hc(n);
+  std::vector<unsigned char> hv(n * e->tsz);
+  const size_t off = (size_t)h.first_slab * e->S * kW;
+  MM_HIP(hipMemcpyAsync(hc.data(), e->cols + off, n * 4, hipMemcpyDeviceToHost, e->stream));
+  MM_HIP(hipMemcpyAsync(hv.data(), (char *)e->vals + off * e->tsz, n * e->tsz, hipMemcpyDeviceToHost,
+                        e->stream));
+  MM_HIP(hipStreamSynchronize(e->stream));
+  for (int row = 0; row < h.nrow; row++) {
+    const size_t sb = (size_t)(row / e->S) * e->S * kW, lane = row % e->S;
+    for (int j = 0; j < kW; j++) {
+      const size_t src = sb + (size_t)j * e->S + lane;
+      cols[(size_t)row * kW + j] = hc[src];
+      vals[(size_t)row * kW + j] =
+          e->fp64 ? ((const double *)hv.data())[src] : (double)((const float *)hv.data())[src];
+    }
+  }
+  return 0;
+}
+
+extern "C" int mm_engine_download_phase(mm_engine *e, uint64_t key, int *elem_type)
+{
+  if (!e || !elem_type) return MM_ERR_ARG;
+  std::lock_guard<std::mutex> lk(e->step_mu);
+  const int s = find_cached(e, key);
+  if (s < 0) return MM_ERR_STATE;
+  const HostSlot &h = e->slots[s];
+  std::vector<uint8_t> m(h.nelm);
+  MM_HIP(hipMemcpyAsync(m.data(), e->elem_mat + h.elem_base, h.nelm, hipMemcpyDeviceToHost, e->stream));
+  MM_HIP(hipStreamSynchronize(e->stream));
+  for (long long i = 0; i < h.nelm; i++) elem_type[i] = m[i];
+  return 0;
+}
+
+extern "C" int mm_engine_download_x(mm_engine *e, uint64_t key, int load, double *x)
+{
+  if (!e || !x) return MM_ERR_ARG;
+  std::lock_guard<std::mutex> lk(e->step_mu);
+  if (load < 0 || load >= e->K) return MM_ERR_ARG;
+  const int s = find_cached(e, key);
+  if (s < 0) return MM_ERR_STATE;
+  const HostSlot &h = e->slots[s];
+  std::vector<double> hx((size_t)h.nrow * e->KS);
+  MM_HIP(hipMemcpyAsync(hx.data(), e->x64 + (size_t)h.first_slab * e->S * e->KS,
+                        hx.size() * sizeof(double), hipMemcpyDeviceToHost, e->stream));
+  MM_HIP(hipStreamSynchronize(e->stream));
+  for (int i = 0; i < h.nrow; i++) x[i] = hx[(size_t)i * e->KS + load];
+  return 0;
+}
+
+extern "C" int mm_engine_time_spmv(mm_engine *e, int reps, double *avg_ms, double *bytes)
+{
+  if (!e || reps < 1) return MM_ERR_ARG;
+  std::lock_guard<std::mutex> lk(e->step_mu);
+  long long hw = e->slabs.high_water();
+  if (hw <= 0) return MM_ERR_STATE;
+  const int grid = (int)hw;
+  EngPtrs P = e->ptrs();
+  // algorithmic bytes of one launch: every resident row reads its W slots
+  // (value + column) and gathers/writes KS-wide vectors
+  double rows = 0.0;
+  for (int s = 0; s < e->rve_cap; s++)
+    if (e->slots[s].state != SL_FREE) rows += e->slots[s].nrow;
+  hipEvent_t a, b;
+  MM_HIP(hipEventCreate(&a));
+  MM_HIP(hipEventCreate(&b));
+  int rc = eng_launch_time(e->fp64, e->K, P, grid, e->stream);  // warm-up
+  if (rc) return rc;
+  MM_HIP(hipEventRecord(a, e->stream));
+  for (int i = 0; i < reps; i++) {
+    rc = eng_launch_time(e->fp64, e->K, P, grid, e->stream);
+    if (rc) return rc;
+  }
+  MM_HIP(hipEventRecord(b, e->stream));
+  MM_HIP(hipEventSynchronize(b));
+  float ms = 0.0f;
+  MM_HIP(hipEventElapsedTime(&ms, a, b));
+  hipEventDestroy(a);
+  hipEventDestroy(b);
+  e->spmv_ms = ms / reps;
+  if (avg_ms) *avg_ms = e->spmv_ms;
+  if (bytes) *bytes = rows * (kW * (double)(e->tsz + 4) + 3.0 * e->K * e->tsz);
+  return 0;
+}

@@ -1,0 +1,138 @@
+// mm_internal.h -- internal declarations shared by the HIP translation units
+// of libmadmax_amd.so.  Not part of the C-ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+
+#include "micro_api.h"
+
+#define MM_HIP(call)                                                                  \
+  do {                                                                                \
+    hipError_t e_ = (call);                                                           \
+    if (e_ != hipSuccess) {                                                           \
+      fprintf(stderr, "madmax_amd: HIP error %s (%d) at %s:%d: %s\n", hipGetErrorName(e_), \
+              (int)e_, __FILE__, __LINE__, #call);                                    \
+      return MM_ERR_HIP;                                                              \
+    }                                                                                 \
+  } while (0)
+
+namespace mm {
+
+constexpr int kW = 81;       // ELL width of the 3D hex8 Jacobian (micro/src/alloc.c:6)
+constexpr int kNd = 24;      // element dofs, 8 nodes x 3
+constexpr int kKe = kNd * kNd;
+constexpr int kNmat = 2;     // MATRIX, FIBER
+
+// ---------------------------------------------------------------------------
+// Boundary classes of a structured hex8 node.  Along each axis a node is at
+// the low face (0), inside (1) or at the high face (2); class = cx + 3cy + 9cz.
+// For every class the ELL column order of a row is the order in which the
+// reference's ell_add_vals first touches each neighbour node (elements in
+// ascending id, element-local node order, src/ell.c:46-84), so it is a fixed
+// list of neighbour offsets.  Each slot also lists the elements that add into
+// it, in ascending element id, with the local node indices a (row node) and b
+// (column node) into that element's Ke.
+struct SlotTables {
+  int nslot[27];                 // neighbour nodes of the class (8..27)
+  uint8_t off[27][27];           // packed (dx+1) + 3(dy+1) + 9(dz+1)
+  uint8_t ncontrib[27][27];      // elements adding into the slot (1..8)
+  uint16_t contrib[27][27][8];   // (ox+1) | (oy+1)<<1 | (oz+1)<<2 | a<<3 | b<<6
+  uint8_t diag_slot[27];         // slot of the node itself
+  uint8_t nelem[27];             // elements around the node (1..8)
+  uint8_t elem[27][8];           // (ox+1) | (oy+1)<<1 | (oz+1)<<2 | a<<3, ascending id
+};
+
+void build_slot_tables(SlotTables *t);
+
+// element-local node index from local coordinates (src/fem.c:324-361 order)
+__host__ __device__ inline int local_node(int lx, int ly, int lz)
+{
+  const int tab[8] = {0, 1, 3, 2, 4, 5, 7, 6};  // index lx + 2ly + 4lz
+  return tab[lx + 2 * ly + 4 * lz];
+}
+__host__ __device__ inline void local_coords(int a, int *lx, int *ly, int *lz)
+{
+  const int cx[8] = {0, 1, 1, 0, 0, 1, 1, 0};
+  const int cy[8] = {0, 0, 1, 1, 0, 0, 1, 1};
+  *lx = cx[a];
+  *ly = cy[a];
+  *lz = a >> 2;
+}
+
+// ---------------------------------------------------------------------------
+// Solver phases of one resident RVE.  Kernel A handles INNER (SpMV + p.q)
+// and XUPD (x += d); kernel B handles RES (FP64 residual + start of a PCG
+// sweep) and INNER (vector updates + r.z); the averaging kernel handles AVG.
+enum Phase : int { PH_FREE = 0, PH_RES = 1, PH_INNER = 2, PH_XUPD = 3, PH_AVG = 4, PH_DONE = 5 };
+
+struct RveDev {
+  // geometry and materials (written at admission)
+  int nex, ney, nez, nx, ny, nz;
+  int nrow, nelm, nslab, first_slab;
+  long long row_base;   // global row of local row 0 (first_slab * slab_rows)
+  long long elem_base;  // offset into the element-material arena
+  double hx, hy, hz, vol;
+  int nfib, tangent_mode, nload, warm;
+  double fx[MM_MAX_FIBERS], fy[MM_MAX_FIBERS], fr[MM_MAX_FIBERS];
+  double young[kNmat], poisson[kNmat];
+  double load[MM_MAX_RHS][MM_NVOI];
+  // solver state
+  int phase, sweep, its, its_total;
+  int cntA, cntB, pbuf, status;
+  unsigned done_in, done_out;
+  double alpha[MM_MAX_RHS], beta[MM_MAX_RHS], rz[MM_MAX_RHS];
+  double bnorm_in[MM_MAX_RHS], r0[MM_MAX_RHS], rnorm[MM_MAX_RHS];
+};
+
+// Results written by the device into host-pinned memory.
+struct RveOut {
+  int phase;   // mirrors RveDev::phase once the RVE is finished
+  int status, its_total, sweeps;
+  double residual_rel;
+  double stress[MM_MAX_RHS][MM_NVOI];
+  double strain[MM_MAX_RHS][MM_NVOI];
+  double c[MM_NVOI * MM_NVOI];
+};
+
+struct SolveParams {
+  int nrhs, slab_rows;
+  double rtol, abs_tol, inner_rtol;
+  int max_inner_its, max_sweeps;
+};
+
+// Device pointers of the engine arena handed to every solver kernel.
+struct EngPtrs {
+  const int *cols;
+  const void *vals;
+  const void *dinv;
+  double *x64;
+  void *d, *r, *z, *pA, *pB, *q;
+  double *part;
+  const int *slab_rve, *slab_row0;
+  RveDev *rves;
+  RveOut *out;
+  const double *ke_tab, *c_tab, *bw_tab;
+  const uint8_t *elem_mat;
+  int S;
+};
+
+// launchers (engine_kernels.hip)
+int upload_slot_tables(hipStream_t s);
+bool eng_supported_rhs(int K);
+int eng_launch_steps(bool fp64, int K, const EngPtrs &P, const SolveParams &sp, int grid, int nsteps,
+                     hipStream_t s);
+int eng_launch_assemble(bool fp64, int K, const EngPtrs &P, int nblk, const int *slab_list, int mode,
+                        hipStream_t s);
+int eng_launch_time(bool fp64, int K, const EngPtrs &P, int grid, hipStream_t s);
+int eng_launch_admit(int n, const int *slots, const RveDev *states, RveDev *rves, int *slab_rve,
+                     int *slab_row0, int S, hipStream_t s);
+int eng_launch_ke(int n, const int *work, const RveDev *rves, double *ke_tab, double *c_tab_out,
+                  double *bw_tab, hipStream_t s);
+int eng_launch_phase(int nblk, const int2 *work, const RveDev *rves, uint8_t *elem_mat, hipStream_t s);
+constexpr int kPartDoubles = 96;
+inline constexpr int vec_stride_host(int K) { return (K == 1) ? 1 : ((K + 1) & ~1); }
+
+}  // namespace mm

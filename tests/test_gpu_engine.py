@@ -1,0 +1,166 @@
+"""GPU parity of the batched / serving engine (mm_engine_*): FP32 ELL slabs
+equal the rounded oracle ELL, batched mixed-precision tangents of uneven RVEs
+match the FP64 oracle within 1e-8, results are the same alone and in a mix,
+resident RVEs are reused without re-assembly, the serving thread admits work
+while other work is on the device, and oversize / invalid requests fail
+cleanly."""
+import time
+
+import numpy as np
+import pytest
+
+from madmax_amd import (MM_PREC_FP64, MM_PREC_MIXED, MM_TANGENT_FD, MM_TANGENT_UNIT, Engine,
+                        configs, fd_loads, rve_spec, unit_loads)
+from madmax_amd import _lib
+from tests.helpers import normwise_rel, oracle_rve
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-8
+GB = 1 << 30
+
+
+def _oracle_tangent(spec, rtol=1e-12):
+    r, _ = oracle_rve(spec)
+    c, _ = r.c_tangent_unit(1e-3, rtol)
+    return c
+
+
+def _solve_batch(eng, specs, key0=0):
+    for i, s in enumerate(specs):
+        assert eng.submit(i + 1, s, unit_loads(), MM_TANGENT_UNIT,
+                          warm_key=(key0 + i + 1) if key0 else 0) == 0
+    eng.drain()
+    res = {r["id"]: r for r in eng.poll()}
+    assert len(res) == len(specs)
+    return [res[i + 1] for i in range(len(specs))]
+
+
+def test_fp32_slabs_equal_rounded_oracle():
+    spec = rve_spec(5, [(0.4, 0.6, 0.3)])
+    ref, et = oracle_rve(spec)
+    cols_ref, vals_ref = ref.assemble()
+    with Engine(nrhs=6, precision=MM_PREC_MIXED, hbm_budget=GB // 4, max_rves=8) as eng:
+        _solve_batch(eng, [spec], key0=100)
+        cols, vals = eng.download_ell(101)
+        assert np.array_equal(eng.download_phase(101), et)
+    assert np.array_equal(cols, cols_ref)
+    assert np.array_equal(vals, vals_ref.astype(np.float32).astype(np.float64))
+
+
+@pytest.mark.parametrize("precision", [MM_PREC_FP64, MM_PREC_MIXED])
+def test_batched_uneven_tangents_vs_oracle(precision):
+    specs = [configs.c4_spec(4, i) for i in range(6)]
+    specs = [s for s in specs if s.nex <= 20][:4] + [rve_spec(7, [(0.45, 0.5, 0.33)])]
+    with Engine(nrhs=6, precision=precision, hbm_budget=2 * GB, max_rves=64,
+                inner_rtol=1e-10 if precision == MM_PREC_FP64 else 1e-5) as eng:
+        out = _solve_batch(eng, specs)
+    for s, r in zip(specs, out):
+        assert r["status"] == 0
+        err = normwise_rel(r["c_tangent"], _oracle_tangent(s))
+        assert err <= TOL, (s.nex, err)
+        c = r["c_tangent"]
+        assert normwise_rel(c, c.T) <= 1e-7  # major symmetry
+
+
+def test_alone_vs_in_mix_identical():
+    target = configs.c2_spec(2, 7, n=12)
+    others = [configs.c4_spec(9, i) for i in range(5)]
+    with Engine(nrhs=6, hbm_budget=2 * GB, max_rves=64) as eng:
+        alone = _solve_batch(eng, [target])[0]
+    with Engine(nrhs=6, hbm_budget=2 * GB, max_rves=64) as eng:
+        mixed = _solve_batch(eng, others[:2] + [target] + others[2:])[2]
+    # per-RVE reductions run over that RVE's slabs only: bitwise identical
+    assert np.array_equal(alone["c_tangent"], mixed["c_tangent"])
+    assert alone["pcg_its"] == mixed["pcg_its"]
+
+
+def test_fd_tangent_seven_loads():
+    spec = rve_spec(6, [(0.5, 0.5, 0.3)])
+    eps = np.array([1e-3, 2e-4, -1e-4, 0.0, 3e-4, 0.0])
+    ref, _ = oracle_rve(spec)
+    c_ref = ref.c_tangent_fd(eps, 0.005, 1e-12)
+    with Engine(nrhs=7, precision=MM_PREC_MIXED, hbm_budget=GB // 2, max_rves=8) as eng:
+        assert eng.submit(1, spec, fd_loads(eps), MM_TANGENT_FD) == 0
+        eng.drain()
+        r = eng.poll()[0]
+    assert r["status"] == 0
+    assert normwise_rel(r["c_tangent"], c_ref) <= TOL
+    out = ref.homog(eps, rtol=1e-12)
+    assert normwise_rel(r["stress"][0], out["stress_ave"]) <= TOL
+
+
+def test_resident_reuse_and_warm_start():
+    spec = rve_spec(10, [(0.5, 0.5, 0.3)])
+    with Engine(nrhs=6, hbm_budget=GB, max_rves=8) as eng:
+        first = _solve_batch(eng, [spec], key0=10)[0]
+        x1 = eng.download_x(11, 0)
+        # same key, cold start: no assembly, same answer
+        assert eng.submit(2, spec, unit_loads(), MM_TANGENT_UNIT, warm_key=11, warm_start=0) == 0
+        eng.drain()
+        cold = eng.poll()[0]
+        # same key, warm start from the kept displacements: converged at once
+        assert eng.submit(3, spec, unit_loads(), MM_TANGENT_UNIT, warm_key=11, warm_start=1) == 0
+        eng.drain()
+        warm = eng.poll()[0]
+        x3 = eng.download_x(11, 0)
+    assert np.array_equal(cold["c_tangent"], first["c_tangent"])
+    assert warm["pcg_its"] < cold["pcg_its"]
+    assert normwise_rel(warm["c_tangent"], first["c_tangent"]) <= 1e-9
+    assert np.max(np.abs(x3 - x1)) <= 1e-9 * np.max(np.abs(x1))
+
+
+def test_serving_thread_admits_while_busy():
+    specs = [configs.c2_spec(5, i, n=8 + 2 * (i % 4)) for i in range(12)]
+    with Engine(nrhs=6, hbm_budget=GB, max_rves=6) as eng:
+        ref = {i + 1: r for i, r in enumerate(_solve_batch(eng, specs))}
+    with Engine(nrhs=6, hbm_budget=GB, max_rves=6, steps_per_launch=8) as eng:
+        eng.start()
+        for i, s in enumerate(specs):
+            assert eng.submit(i + 1, s, unit_loads(), MM_TANGENT_UNIT) == 0
+            time.sleep(0.002)
+        got = {}
+        t0 = time.time()
+        while len(got) < len(specs) and time.time() - t0 < 60:
+            for r in eng.poll():
+                got[r["id"]] = r
+            time.sleep(0.001)
+        assert eng.stop() == 0
+    assert len(got) == len(specs)
+    overlap = any(got[i]["t_admit"] < max(got[j]["t_done"] for j in got if j != i)
+                  and got[i]["t_done"] > min(got[j]["t_admit"] for j in got if j != i) for i in got)
+    assert overlap
+    for i in got:
+        assert normwise_rel(got[i]["c_tangent"], ref[i]["c_tangent"]) <= 1e-12
+
+
+def test_admission_limits():
+    with Engine(nrhs=6, hbm_budget=64 << 20, max_rves=4) as eng:
+        big = rve_spec(40)
+        assert eng.submit(1, big, unit_loads(), MM_TANGENT_UNIT) == _lib.MM_REJECTED
+        bad = rve_spec(4, young=(float("nan"), 1e8))
+        assert eng.submit(2, bad, unit_loads(), MM_TANGENT_UNIT) == _lib.MM_ERR_ARG
+        loads = unit_loads()
+        loads[2] = np.array([0, 0, float("inf"), 0, 0, 0])
+        assert eng.submit(3, rve_spec(4), loads, MM_TANGENT_UNIT) == _lib.MM_ERR_ARG
+        # more RVEs than slots: admitted as slots free up
+        specs = [rve_spec(6, [(0.5, 0.5, 0.1 + 0.02 * i)]) for i in range(10)]
+        out = _solve_batch(eng, specs)
+        assert all(r["status"] == 0 for r in out)
+
+
+def test_not_converged_status():
+    with Engine(nrhs=6, hbm_budget=GB // 4, max_rves=4, max_inner_its=2, max_sweeps=1) as eng:
+        assert eng.submit(1, rve_spec(8, [(0.5, 0.5, 0.3)]), unit_loads(), MM_TANGENT_UNIT) == 0
+        eng.drain()
+        r = eng.poll()[0]
+    assert r["status"] == _lib.MM_NOT_CONVERGED
+
+
+def test_c2_tangent_30cube_vs_oracle():
+    spec = configs.c2_spec(2, 0)
+    c_ref = _oracle_tangent(spec, rtol=1e-11)
+    with Engine(nrhs=6, precision=MM_PREC_MIXED, hbm_budget=4 * GB, max_rves=8) as eng:
+        r = _solve_batch(eng, [spec])[0]
+    assert r["status"] == 0
+    assert normwise_rel(r["c_tangent"], c_ref) <= TOL
