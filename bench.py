@@ -49,7 +49,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--batch", type=int, default=128)
     ap.add_argument("--n", type=int, default=30)
-    ap.add_argument("--slab", type=int, default=256)
+    ap.add_argument("--slab", type=int, default=128, help="nodes per ELL slab (64, 128, 256)")
     ap.add_argument("--inner-rtol", type=float, default=1e-5)
     ap.add_argument("--spl", type=int, default=16, help="PCG steps per graph launch")
     ap.add_argument("--cpu-sample", type=int, default=2, help="RVE tangents timed on the CPU")
@@ -72,7 +72,7 @@ def main():
     budget = int(per_rve * B * 1.3) + (256 << 20)
     log("batch %d x %d^3 (%d rows each), arena %.1f GB" % (B, n, rows, budget / 1e9))
     eng = mm.Engine(nrhs=6, precision=mm.MM_PREC_MIXED, hbm_budget=budget, max_rves=B + 8,
-                    slab_rows=args.slab, inner_rtol=args.inner_rtol, rtol=1e-10,
+                    slab_nodes=args.slab, inner_rtol=args.inner_rtol, rtol=1e-10,
                     steps_per_launch=args.spl, use_graph=True)
     for w in range(args.warmup):
         t = time.perf_counter()
@@ -102,7 +102,7 @@ def main():
     plain = None
     if not args.no_plain:
         pe = mm.Engine(nrhs=1, precision=mm.MM_PREC_FP64, hbm_budget=per_rve * 2 + (64 << 20),
-                       max_rves=2, slab_rows=256, inner_rtol=1e-10, rtol=0.0, abs_tol=1e-4,
+                       max_rves=2, slab_nodes=128, inner_rtol=1e-10, rtol=0.0, abs_tol=1e-4,
                        max_sweeps=2, sync_every_step=True, use_graph=False)
         nplain = 2
         t = time.perf_counter()
@@ -148,7 +148,7 @@ def main():
     if args.serve > 0:
         stream = configs.c5_stream(count=args.serve, rate=4000.0)
         se = mm.Engine(nrhs=7, precision=mm.MM_PREC_MIXED, hbm_budget=8 << 30, max_rves=512,
-                       slab_rows=args.slab, inner_rtol=args.inner_rtol, steps_per_launch=8)
+                       slab_nodes=args.slab, inner_rtol=args.inner_rtol, steps_per_launch=8)
         se.start()
         t0s = time.perf_counter()
         got = []
@@ -185,7 +185,7 @@ def main():
         "data": "synthetic (seeded two-phase fiber RVEs)",
         "config": {"workload": "configs[1]: %d x %d^3 hex8 RVE tangents per step, 6-RHS block, "
                                "FP32 PCG + FP64 IR" % (B, n), "rve_elements": n ** 3,
-                   "batch": B, "slab_rows": args.slab, "parallelism": "single GPU"},
+                   "batch": B, "slab_nodes": args.slab, "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
                      "kernel": "k_spmv_time<float,6> (batched ELL SpMV, %d resident RVEs)" % B,

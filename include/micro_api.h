@@ -118,7 +118,8 @@ typedef struct {
   int device;
   size_t hbm_budget;     /* bytes for the RVE arena (0 = 80% of free memory) */
   int max_rves;          /* resident RVE slots (0 = 4096) */
-  int slab_rows;         /* rows per ELL slab, multiple of 64 (0 = autotuned) */
+  int slab_nodes;        /* nodes per ELL slab = lanes per workgroup: 64, 128 or 256
+                            (0 = 128); a slab holds 3 * slab_nodes rows */
   int nrhs;              /* load cases per RVE, 1..MM_MAX_RHS */
   int precision;         /* MM_PREC_* */
   double rtol;           /* FP64 residual target relative to the first residual */
@@ -183,7 +184,7 @@ typedef struct {
   long long completed;
   long long steps;
   size_t arena_bytes, arena_used;
-  int slab_rows;
+  int slab_nodes;
   double spmv_ms;     /* average duration of the ELL SpMV kernel, from HIP events */
 } mm_engine_info;
 int mm_engine_info_get(mm_engine *eng, mm_engine_info *info);

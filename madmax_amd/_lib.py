@@ -58,7 +58,7 @@ class MicroStats(C.Structure):
 
 class EngineConfig(C.Structure):
     _fields_ = [("device", C.c_int), ("hbm_budget", C.c_size_t), ("max_rves", C.c_int),
-                ("slab_rows", C.c_int), ("nrhs", C.c_int), ("precision", C.c_int),
+                ("slab_nodes", C.c_int), ("nrhs", C.c_int), ("precision", C.c_int),
                 ("rtol", C.c_double), ("abs_tol", C.c_double), ("inner_rtol", C.c_double),
                 ("max_inner_its", C.c_int), ("max_sweeps", C.c_int),
                 ("steps_per_launch", C.c_int), ("use_graph", C.c_int),
@@ -83,7 +83,7 @@ class Result(C.Structure):
 class EngineInfo(C.Structure):
     _fields_ = [("resident", C.c_int), ("queued", C.c_int), ("completed", C.c_longlong),
                 ("steps", C.c_longlong), ("arena_bytes", C.c_size_t),
-                ("arena_used", C.c_size_t), ("slab_rows", C.c_int), ("spmv_ms", C.c_double)]
+                ("arena_used", C.c_size_t), ("slab_nodes", C.c_int), ("spmv_ms", C.c_double)]
 
 
 # name -> (restype, argtypes); every symbol declared in include/*.h

@@ -149,19 +149,19 @@ int ensure_device(T **p, size_t *cap, size_t n)
 
 struct mm_engine {
   mm_engine_config cfg;
-  int S = 256, K = 1, KS = 1;
+  int SN = 128, K = 1, KS = 1;
   bool fp64 = false;
   size_t tsz = 4;
   hipStream_t stream = nullptr;
   // arena
   long long nslab_cap = 0, elem_cap = 0;
   int rve_cap = 0;
-  int *cols = nullptr;
+  int *bcols = nullptr;
   void *vals = nullptr, *dinv = nullptr;
   double *x64 = nullptr;
-  void *d = nullptr, *r = nullptr, *z = nullptr, *pA = nullptr, *pB = nullptr, *q = nullptr;
+  void *d = nullptr, *r = nullptr, *z = nullptr, *p = nullptr, *q = nullptr;
   double *part = nullptr;
-  int *slab_rve = nullptr, *slab_row0 = nullptr;
+  int *slab_rve = nullptr, *slab_node0 = nullptr;
   RveDev *rves = nullptr;
   uint8_t *elem_mat = nullptr;
   double *ke_tab = nullptr, *c_tab = nullptr, *bw_tab = nullptr;
@@ -204,33 +204,32 @@ struct mm_engine {
   EngPtrs ptrs() const
   {
     EngPtrs P;
-    P.cols = cols;
+    P.bcols = bcols;
     P.vals = vals;
     P.dinv = dinv;
     P.x64 = x64;
     P.d = d;
     P.r = r;
     P.z = z;
-    P.pA = pA;
-    P.pB = pB;
+    P.p = p;
     P.q = q;
     P.part = part;
     P.slab_rve = slab_rve;
-    P.slab_row0 = slab_row0;
+    P.slab_node0 = slab_node0;
     P.rves = rves;
     P.out = out_dev;
     P.ke_tab = ke_tab;
     P.c_tab = c_tab;
     P.bw_tab = bw_tab;
     P.elem_mat = elem_mat;
-    P.S = S;
+    P.SN = SN;
     return P;
   }
   SolveParams params() const
   {
     SolveParams sp;
     sp.nrhs = K;
-    sp.slab_rows = S;
+    sp.slab_nodes = SN;
     sp.rtol = cfg.rtol;
     sp.abs_tol = cfg.abs_tol;
     sp.inner_rtol = cfg.inner_rtol;
@@ -242,17 +241,19 @@ struct mm_engine {
   {
     return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   }
+  int rows_per_slab() const { return 3 * SN; }
   size_t bytes_per_slab() const
   {
-    return (size_t)S * kW * (tsz + 4) + (size_t)S * (KS * (8 + 6 * tsz) + tsz) +
-           kPartDoubles * 8 + 8 + (size_t)S / 3 + 1;
+    const size_t RS = (size_t)rows_per_slab();
+    return RS * kW * tsz + (size_t)SN * 27 * 4 + RS * (KS * (8 + 5 * tsz) + tsz) +
+           kPartDoubles * 8 + 8 + (size_t)SN + 1;
   }
 };
 
 static int engine_free_arena(mm_engine *e)
 {
-  void *ptrs[] = {e->cols, e->vals, e->dinv, e->x64, e->d, e->r, e->z, e->pA, e->pB, e->q, e->part,
-                  e->slab_rve, e->slab_row0, e->rves, e->elem_mat, e->ke_tab, e->c_tab, e->bw_tab,
+  void *ptrs[] = {e->bcols, e->vals, e->dinv, e->x64, e->d, e->r, e->z, e->p, e->q, e->part,
+                  e->slab_rve, e->slab_node0, e->rves, e->elem_mat, e->ke_tab, e->c_tab, e->bw_tab,
                   e->dv_states, e->dv_slots, e->dv_ke, e->dv_slabs, e->dv_phase};
   for (void *p : ptrs)
     if (p) hipFree(p);
@@ -269,7 +270,7 @@ extern "C" int mm_engine_config_default(mm_engine_config *c)
   c->device = 0;
   c->hbm_budget = 0;
   c->max_rves = 0;
-  c->slab_rows = 0;
+  c->slab_nodes = 0;
   c->nrhs = 6;
   c->precision = MM_PREC_MIXED;
   c->rtol = 1.0e-10;
@@ -290,7 +291,8 @@ extern "C" int mm_engine_create(const mm_engine_config *cfg, mm_engine **out)
     fprintf(stderr, "madmax_amd: nrhs %d not supported (1, 6 or 7)\n", cfg->nrhs);
     return MM_ERR_ARG;
   }
-  if (cfg->slab_rows != 0 && (cfg->slab_rows % 64 != 0 || cfg->slab_rows > 512)) return MM_ERR_ARG;
+  if (cfg->slab_nodes != 0 && cfg->slab_nodes != 64 && cfg->slab_nodes != 128 && cfg->slab_nodes != 256)
+    return MM_ERR_ARG;
   if (cfg->precision != MM_PREC_FP64 && cfg->precision != MM_PREC_MIXED) return MM_ERR_ARG;
   if (!(cfg->rtol >= 0) || !(cfg->inner_rtol > 0) || cfg->max_inner_its < 1 || cfg->max_sweeps < 1)
     return MM_ERR_ARG;
@@ -308,7 +310,7 @@ extern "C" int mm_engine_create(const mm_engine_config *cfg, mm_engine **out)
     e->cfg.steps_per_launch = 1;
     e->cfg.use_graph = 0;
   }
-  e->S = cfg->slab_rows ? cfg->slab_rows : 256;
+  e->SN = cfg->slab_nodes ? cfg->slab_nodes : 128;
   e->K = cfg->nrhs;
   e->KS = vec_stride_host(e->K);
   e->fp64 = (cfg->precision == MM_PREC_FP64);
@@ -331,8 +333,8 @@ extern "C" int mm_engine_create(const mm_engine_config *cfg, mm_engine **out)
     return MM_ERR_ALLOC;
   }
   e->nslab_cap = (long long)((budget - (size_t)e->rve_cap * per_rve) / e->bytes_per_slab());
-  e->elem_cap = e->nslab_cap * e->S / 3 + 64;
-  const size_t nrows = (size_t)e->nslab_cap * e->S;
+  e->elem_cap = e->nslab_cap * e->SN + 64;
+  const size_t nrows = (size_t)e->nslab_cap * e->rows_per_slab();
   const size_t slots = nrows * kW;
   if (hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking) != hipSuccess) {
     delete e;
@@ -348,19 +350,18 @@ extern "C" int mm_engine_create(const mm_engine_config *cfg, mm_engine **out)
     }                                                                       \
     e->arena_bytes += (bytes);                                              \
   } while (0)
-  EALLOC(e->cols, slots * 4);
+  EALLOC(e->bcols, (size_t)e->nslab_cap * e->SN * 27 * 4);
   EALLOC(e->vals, slots * e->tsz);
   EALLOC(e->dinv, nrows * e->tsz);
   EALLOC(e->x64, nrows * e->KS * 8);
   EALLOC(e->d, nrows * e->KS * e->tsz);
   EALLOC(e->r, nrows * e->KS * e->tsz);
   EALLOC(e->z, nrows * e->KS * e->tsz);
-  EALLOC(e->pA, nrows * e->KS * e->tsz);
-  EALLOC(e->pB, nrows * e->KS * e->tsz);
+  EALLOC(e->p, nrows * e->KS * e->tsz);
   EALLOC(e->q, nrows * e->KS * e->tsz);
   EALLOC(e->part, (size_t)e->nslab_cap * kPartDoubles * 8);
   EALLOC(e->slab_rve, (size_t)e->nslab_cap * 4);
-  EALLOC(e->slab_row0, (size_t)e->nslab_cap * 4);
+  EALLOC(e->slab_node0, (size_t)e->nslab_cap * 4);
   EALLOC(e->rves, (size_t)e->rve_cap * sizeof(RveDev));
   EALLOC(e->elem_mat, (size_t)e->elem_cap);
   EALLOC(e->ke_tab, (size_t)e->rve_cap * kNmat * kKe * 8);
@@ -417,7 +418,7 @@ static void footprint(const mm_engine *e, const mm_rve_spec &s, int *nrow, long 
 {
   const long long nn = (long long)(s.nex + 1) * (s.ney + 1) * (s.nez + 1);
   *nrow = (int)(3 * nn);
-  *nslab = (3 * nn + e->S - 1) / e->S;
+  *nslab = (nn + e->SN - 1) / e->SN;
   *nelm = (long long)s.nex * s.ney * s.nez;
 }
 
@@ -486,7 +487,7 @@ static void fill_state(const mm_engine *e, const HostSlot &h, RveDev *st, int wa
   st->nelm = (int)h.nelm;
   st->nslab = (int)h.nslab;
   st->first_slab = (int)h.first_slab;
-  st->row_base = h.first_slab * e->S;
+  st->row_base = h.first_slab * e->rows_per_slab();
   st->elem_base = h.elem_base;
   st->hx = s.lx / s.nex;
   st->hy = s.ly / s.ney;
@@ -624,7 +625,7 @@ static int admit(mm_engine *e)
     MM_HIP(hipMemcpyAsync(e->dv_slabs, e->st_slabs, (nsl + nsl_re) * sizeof(int), hipMemcpyHostToDevice,
                           e->stream));
   EngPtrs P = e->ptrs();
-  if ((rc = eng_launch_admit((int)n, e->dv_slots, e->dv_states, e->rves, e->slab_rve, e->slab_row0, e->S,
+  if ((rc = eng_launch_admit((int)n, e->dv_slots, e->dv_states, e->rves, e->slab_rve, e->slab_node0, e->SN,
                              e->stream)))
     return rc;
   if ((rc = eng_launch_ke((int)nke, e->dv_ke, e->rves, e->ke_tab, e->c_tab, e->bw_tab, e->stream)))
@@ -832,7 +833,7 @@ extern "C" int mm_engine_info_get(mm_engine *e, mm_engine_info *info)
                          ? (size_t)((double)e->arena_bytes * (double)(e->slabs.cap - free_slabs) /
                                     (double)e->slabs.cap)
                          : 0;
-  info->slab_rows = e->S;
+  info->slab_nodes = e->SN;
   info->spmv_ms = e->spmv_ms;
   return 0;
 }
@@ -863,19 +864,24 @@ extern "C" int mm_engine_download_ell(mm_engine *e, uint64_t key, int *cols, dou
   const int s = find_cached(e, key);
   if (s < 0) return MM_ERR_STATE;
   const HostSlot &h = e->slots[s];
-  const size_t n = (size_t)h.nslab * e->S * kW;
-  std::vector<int> hc(n);
-  std::vector<unsigned char> hv(n * e->tsz);
-  const size_t off = (size_t)h.first_slab * e->S * kW;
-  MM_HIP(hipMemcpyAsync(hc.data(), e->cols + off, n * 4, hipMemcpyDeviceToHost, e->stream));
-  MM_HIP(hipMemcpyAsync(hv.data(), (char *)e->vals + off * e->tsz, n * e->tsz, hipMemcpyDeviceToHost,
-                        e->stream));
+  const size_t RS = (size_t)e->rows_per_slab();
+  const size_t nv = (size_t)h.nslab * RS * kW, nc = (size_t)h.nslab * e->SN * 27;
+  std::vector<int> hc(nc);
+  std::vector<unsigned char> hv(nv * e->tsz);
+  MM_HIP(hipMemcpyAsync(hc.data(), e->bcols + (size_t)h.first_slab * e->SN * 27, nc * 4,
+                        hipMemcpyDeviceToHost, e->stream));
+  MM_HIP(hipMemcpyAsync(hv.data(), (char *)e->vals + (size_t)h.first_slab * RS * kW * e->tsz,
+                        nv * e->tsz, hipMemcpyDeviceToHost, e->stream));
   MM_HIP(hipStreamSynchronize(e->stream));
+  // the reference's column of slot 3t + d' of row 3n + d is 3 * bcols[n][t] + d'
   for (int row = 0; row < h.nrow; row++) {
-    const size_t sb = (size_t)(row / e->S) * e->S * kW, lane = row % e->S;
+    const int n = row / 3, d = row % 3;
+    const size_t sl = (size_t)(n / e->SN), lane = n % e->SN;
     for (int j = 0; j < kW; j++) {
-      const size_t src = sb + (size_t)j * e->S + lane;
-      cols[(size_t)row * kW + j] = hc[src];
+      const int t = j / 3, dd = j % 3;
+      const int m = hc[sl * e->SN * 27 + (size_t)t * e->SN + lane];
+      cols[(size_t)row * kW + j] = (m < 0) ? -1 : 3 * m + dd;
+      const size_t src = sl * RS * kW + (size_t)j * RS + 3 * lane + d;
       vals[(size_t)row * kW + j] =
           e->fp64 ? ((const double *)hv.data())[src] : (double)((const float *)hv.data())[src];
     }
@@ -906,7 +912,7 @@ extern "C" int mm_engine_download_x(mm_engine *e, uint64_t key, int load, double
   if (s < 0) return MM_ERR_STATE;
   const HostSlot &h = e->slots[s];
   std::vector<double> hx((size_t)h.nrow * e->KS);
-  MM_HIP(hipMemcpyAsync(hx.data(), e->x64 + (size_t)h.first_slab * e->S * e->KS,
+  MM_HIP(hipMemcpyAsync(hx.data(), e->x64 + (size_t)h.first_slab * e->rows_per_slab() * e->KS,
                         hx.size() * sizeof(double), hipMemcpyDeviceToHost, e->stream));
   MM_HIP(hipStreamSynchronize(e->stream));
   for (int i = 0; i < h.nrow; i++) x[i] = hx[(size_t)i * e->KS + load];
@@ -918,11 +924,12 @@ extern "C" int mm_engine_time_spmv(mm_engine *e, int reps, double *avg_ms, doubl
   if (!e || reps < 1) return MM_ERR_ARG;
   std::lock_guard<std::mutex> lk(e->step_mu);
   long long hw = e->slabs.high_water();
-  if (hw <= 0) return MM_ERR_STATE;
+  if (hw <= 0 || e->n_active > 0) return MM_ERR_STATE;  // it overwrites q
   const int grid = (int)hw;
   EngPtrs P = e->ptrs();
-  // algorithmic bytes of one launch: every resident row reads its W slots
-  // (value + column) and gathers/writes KS-wide vectors
+  // algorithmic bytes of one launch: every resident row reads its 81 values,
+  // every node its 27 block columns, and the vector z is read and w written
+  // once per row and load case
   double rows = 0.0;
   for (int s = 0; s < e->rve_cap; s++)
     if (e->slots[s].state != SL_FREE) rows += e->slots[s].nrow;
@@ -944,6 +951,6 @@ extern "C" int mm_engine_time_spmv(mm_engine *e, int reps, double *avg_ms, doubl
   hipEventDestroy(b);
   e->spmv_ms = ms / reps;
   if (avg_ms) *avg_ms = e->spmv_ms;
-  if (bytes) *bytes = rows * (kW * (double)(e->tsz + 4) + 3.0 * e->K * e->tsz);
+  if (bytes) *bytes = rows * (kW * (double)e->tsz + 27.0 * 4.0 / 3.0 + 2.0 * e->K * e->tsz);
   return 0;
 }

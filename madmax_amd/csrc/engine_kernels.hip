@@ -1,19 +1,27 @@
 // engine_kernels.hip -- CDNA4 kernels of the RVE engine.
 //
-// Arena layout in HBM (S = slab rows, W = 81, KS = padded RHS count):
-//   vals/cols : [slab][W][S]   column-major ELL slabs, RVE-local column ids
-//                              (bit-identical to the reference's cols)
-//   vectors   : [row][KS]      x64 (FP64 displacement), d (correction),
-//                              r, z, pA/pB (ping-pong search direction), q
-//   dinv      : [row]          Jacobi preconditioner
-//   part      : [slab][96]     per-block partial sums (FP64)
-// Every resident RVE owns a contiguous run of slabs; row_base = first_slab*S.
+// Arena layout in HBM.  Rows (dofs) are numbered 3*node + d; a slab holds SN
+// consecutive nodes (3*SN rows) of one RVE and is processed by one workgroup
+// of SN lanes, one lane per node:
+//   vals  : [slab][81][3*SN]  ELL values in the reference's slot order
+//                             (slot j = 3t + d' of row 3n + d), column-major
+//                             inside the slab so lane i reads the three rows
+//                             of its node as 12 contiguous bytes
+//   bcols : [slab][27][SN]    RVE-local node id of slot block t (-1 = empty);
+//                             the reference's column of slot 3t + d' is
+//                             3*bcols + d', shared by the node's three rows
+//   vectors [row][KS]         x64 (FP64 displacement), d (correction), r, z,
+//                             p, q (PCG), KS = load cases padded to even
+//   dinv  : [row]             Jacobi preconditioner
+//   part  : [slab][96]        per-workgroup partial sums (FP64)
+// Every resident RVE owns a contiguous run of slabs; its first row is
+// first_slab * 3 * SN.
 //
-// Kernel A and kernel B advance every resident RVE by one phase step; their
-// grids cover all slabs, and each block reads the phase of its RVE.  The
-// block that finishes last for an RVE (arrive_last) reduces the partials of
-// that RVE in slab order, so results do not depend on which other RVEs share
-// the launch.
+// The solver kernels advance every resident RVE by one phase step; their grid
+// covers all slabs and each workgroup reads the phase of its RVE.  The
+// workgroup that finishes last for an RVE (arrive_last) reduces that RVE's
+// partials in slab order, so an RVE's results do not depend on which other
+// RVEs share the launch.
 #include <hip/hip_runtime.h>
 
 #include "mm_device.h"
@@ -32,15 +40,23 @@ int upload_slot_tables(hipStream_t s)
   return 0;
 }
 
-constexpr int kPart = 96;  // doubles of partials per slab
+constexpr int kPart = kPartDoubles;
 
 __host__ __device__ constexpr int vec_stride(int K) { return (K == 1) ? 1 : ((K + 1) & ~1); }
+
+__device__ __forceinline__ int node_class(int ix, int iy, int iz, int nx, int ny, int nz)
+{
+  const int cx = (ix == 0) ? 0 : ((ix == nx - 1) ? 2 : 1);
+  const int cy = (iy == 0) ? 0 : ((iy == ny - 1) ? 2 : 1);
+  const int cz = (iz == 0) ? 0 : ((iz == nz - 1) ? 2 : 1);
+  return cx + 3 * cy + 9 * cz;
+}
 
 // ---------------------------------------------------------------------------
 // Admission: scatter new RVE states and their slab ownership.
 __global__ void k_admit(int n, const int *__restrict__ slots, const RveDev *__restrict__ states,
                         RveDev *__restrict__ rves, int *__restrict__ slab_rve,
-                        int *__restrict__ slab_row0, int S)
+                        int *__restrict__ slab_node0, int SN)
 {
   const int i = blockIdx.x;
   if (i >= n) return;
@@ -49,7 +65,7 @@ __global__ void k_admit(int n, const int *__restrict__ slots, const RveDev *__re
   if (threadIdx.x == 0) rves[slot] = st;
   for (int s = threadIdx.x; s < st.nslab; s += blockDim.x) {
     slab_rve[st.first_slab + s] = slot;
-    slab_row0[st.first_slab + s] = s * S;
+    slab_node0[st.first_slab + s] = s * SN;
   }
 }
 
@@ -165,20 +181,21 @@ __global__ void k_phase(int nblk, const int2 *__restrict__ work, const RveDev *_
   elem_mat[R.elem_base + e] = m;
 }
 
-// ELL assembly: one lane per row.  Writes all W slots of the row (column ids
-// and values, -1 / 0 in the tail), the Jacobi diagonal inverse and the
-// initial displacement (affine on the boundary, micro/src/homogenize.c:
-// 227-242, 335-342).  Each slot value is the sum of the Ke entries of the
-// elements sharing the two nodes, in ascending element id, the first one
-// stored and the rest added (src/ell.c:46-73).  Boundary rows become identity
-// rows; boundary columns are kept (src/ell.c:86-118).
+// ELL assembly: one lane per node, writing the three rows of the node.  Every
+// slot value is the sum of the Ke entries of the elements sharing the two
+// nodes, in ascending element id, the first stored and the rest added
+// (src/ell.c:46-73, micro/src/assembly.c:93-110).  Boundary rows become
+// identity rows with their columns kept (src/ell.c:86-118); the empty tail
+// holds -1 / 0.  Also writes the Jacobi diagonal inverse and the initial
+// displacement: affine on the boundary (micro/src/homogenize.c:227-242,
+// 335-342), zero or the kept values inside.
 // mode 0: full assembly; mode 1: displacement only (resident matrix reused).
 template <typename TV, int K>
 __global__ void k_assemble(int nblk, const int *__restrict__ slab_list, const int *__restrict__ slab_rve,
-                           const int *__restrict__ slab_row0, const RveDev *__restrict__ rves,
+                           const int *__restrict__ slab_node0, const RveDev *__restrict__ rves,
                            const double *__restrict__ ke_tab, const uint8_t *__restrict__ elem_mat,
-                           int *__restrict__ cols, TV *__restrict__ vals, TV *__restrict__ dinv,
-                           double *__restrict__ x64, int S, int mode)
+                           int *__restrict__ bcols, TV *__restrict__ vals, TV *__restrict__ dinv,
+                           double *__restrict__ x64, int SN, int mode)
 {
 #pragma clang fp contract(off)
   constexpr int KS = vec_stride(K);
@@ -187,85 +204,82 @@ __global__ void k_assemble(int nblk, const int *__restrict__ slab_list, const in
   const int slot = slab_rve[slab];
   const RveDev &R = rves[slot];
   const int tid = threadIdx.x;
-  const int row = slab_row0[slab] + tid;
-  const long long sb = (long long)slab * S * kW;
-  const long long grow = R.row_base + row;
-  if (row >= R.nrow) {
+  const int n = slab_node0[slab] + tid;
+  const int RS = 3 * SN;
+  const long long vb = (long long)slab * kW * RS;
+  const long long cb = (long long)slab * 27 * SN;
+  const long long grow0 = R.row_base + 3LL * n;
+  const int nn = R.nrow / 3;
+  if (n >= nn) {
     if (mode == 0) {
-      for (int j = 0; j < kW; j++) {
-        cols[sb + (long long)j * S + tid] = -1;
-        vals[sb + (long long)j * S + tid] = (TV)0;
-      }
-      dinv[grow] = (TV)0;
+      for (int t = 0; t < 27; t++) bcols[cb + (long long)t * SN + tid] = -1;
+      for (int j = 0; j < kW; j++)
+        for (int d = 0; d < 3; d++) vals[vb + (long long)j * RS + 3 * tid + d] = (TV)0;
+      for (int d = 0; d < 3; d++) dinv[grow0 + d] = (TV)0;
     }
-    for (int k = 0; k < KS; k++) x64[grow * KS + k] = 0.0;
+    for (int d = 0; d < 3; d++)
+      for (int k = 0; k < KS; k++) x64[(grow0 + d) * KS + k] = 0.0;
     return;
   }
-  const int n = row / 3, d = row - 3 * (row / 3);
   const int nx = R.nx, nxy = R.nx * R.ny;
   const int ix = n % nx, iy = (n / nx) % R.ny, iz = n / nxy;
-  const int cx = (ix == 0) ? 0 : ((ix == nx - 1) ? 2 : 1);
-  const int cy = (iy == 0) ? 0 : ((iy == R.ny - 1) ? 2 : 1);
-  const int cz = (iz == 0) ? 0 : ((iz == R.nz - 1) ? 2 : 1);
-  const int cls = cx + 3 * cy + 9 * cz;
+  const int cls = node_class(ix, iy, iz, nx, R.ny, R.nz);
   const bool bc = (cls != 13);
   if (mode == 0) {
     const int ns = g_slots.nslot[cls];
     const int dslot = g_slots.diag_slot[cls];
     const double *ke0 = ke_tab + (size_t)slot * kNmat * kKe;
-    double diag = 0.0;
+    double diag[3] = {0.0, 0.0, 0.0};
     for (int t = 0; t < 27; t++) {
       if (t < ns) {
         const int key = g_slots.off[cls][t];
         const int m = n + (key % 3 - 1) + ((key / 3) % 3 - 1) * nx + (key / 9 - 1) * nxy;
-        const int nc = g_slots.ncontrib[cls][t];
-        double v[3];
+        double v[3][3];  // [d][d']
         if (bc) {
-          for (int dd = 0; dd < 3; dd++) v[dd] = (3 * m + dd == row) ? 1.0 : 0.0;
+          for (int d = 0; d < 3; d++)
+            for (int dd = 0; dd < 3; dd++) v[d][dd] = (m == n && d == dd) ? 1.0 : 0.0;
         } else {
-          for (int dd = 0; dd < 3; dd++) v[dd] = 0.0;
+          const int nc = g_slots.ncontrib[cls][t];
           for (int c = 0; c < nc; c++) {
             const int code = g_slots.contrib[cls][t][c];
             const int ox = (code & 1) - 1, oy = ((code >> 1) & 1) - 1, oz = ((code >> 2) & 1) - 1;
             const int a = (code >> 3) & 7, b = (code >> 6) & 7;
             const int e = (ix + ox) + (iy + oy) * R.nex + (iz + oz) * R.nex * R.ney;
-            const double *ke = ke0 + (size_t)elem_mat[R.elem_base + e] * kKe + (3 * a + d) * kNd + 3 * b;
-            for (int dd = 0; dd < 3; dd++) v[dd] = (c == 0) ? ke[dd] : v[dd] + ke[dd];
+            const double *ke = ke0 + (size_t)elem_mat[R.elem_base + e] * kKe + 3 * a * kNd + 3 * b;
+            for (int d = 0; d < 3; d++)
+              for (int dd = 0; dd < 3; dd++)
+                v[d][dd] = (c == 0) ? ke[d * kNd + dd] : v[d][dd] + ke[d * kNd + dd];
           }
         }
-        if (t == dslot) diag = v[d];
-        for (int dd = 0; dd < 3; dd++) {
-          cols[sb + (long long)(3 * t + dd) * S + tid] = 3 * m + dd;
-          vals[sb + (long long)(3 * t + dd) * S + tid] = (TV)v[dd];
-        }
+        if (t == dslot)
+          for (int d = 0; d < 3; d++) diag[d] = v[d][d];
+        bcols[cb + (long long)t * SN + tid] = m;
+        for (int dd = 0; dd < 3; dd++)
+          for (int d = 0; d < 3; d++) vals[vb + (long long)(3 * t + dd) * RS + 3 * tid + d] = (TV)v[d][dd];
       } else {
-        for (int dd = 0; dd < 3; dd++) {
-          cols[sb + (long long)(3 * t + dd) * S + tid] = -1;
-          vals[sb + (long long)(3 * t + dd) * S + tid] = (TV)0;
-        }
+        bcols[cb + (long long)t * SN + tid] = -1;
+        for (int dd = 0; dd < 3; dd++)
+          for (int d = 0; d < 3; d++) vals[vb + (long long)(3 * t + dd) * RS + 3 * tid + d] = (TV)0;
       }
     }
-    dinv[grow] = (TV)(1.0 / diag);
+    for (int d = 0; d < 3; d++) dinv[grow0 + d] = (TV)(1.0 / diag[d]);
   }
   // initial displacement for every load case
   const double X[3] = {ix * R.hx, iy * R.hy, iz * R.hz};
-  for (int k = 0; k < KS; k++) {
-    double u = 0.0;
-    if (k < R.nload && bc) {
-      const double *s = R.load[k];
-      if (d == 0) u = s[0] * X[0] + s[3] / 2 * X[1] + s[5] / 2 * X[2];
-      else if (d == 1) u = s[3] / 2 * X[0] + s[1] * X[1] + s[4] / 2 * X[2];
-      else u = s[5] / 2 * X[0] + s[4] / 2 * X[1] + s[2] * X[2];
-      x64[grow * KS + k] = u;
-    } else if (!(R.warm && k < R.nload)) {
-      x64[grow * KS + k] = 0.0;
+  for (int d = 0; d < 3; d++)
+    for (int k = 0; k < KS; k++) {
+      if (k < R.nload && bc) {
+        const double *s = R.load[k];
+        double u;
+        if (d == 0) u = s[0] * X[0] + s[3] / 2 * X[1] + s[5] / 2 * X[2];
+        else if (d == 1) u = s[3] / 2 * X[0] + s[1] * X[1] + s[4] / 2 * X[2];
+        else u = s[5] / 2 * X[0] + s[4] / 2 * X[1] + s[2] * X[2];
+        x64[(grow0 + d) * KS + k] = u;
+      } else if (!(R.warm && k < R.nload)) {
+        x64[(grow0 + d) * KS + k] = 0.0;
+      }
     }
-  }
 }
-
-// p = z + beta p_old, evaluated identically wherever it is needed
-template <typename TS>
-__device__ __forceinline__ TS pform(TS z, TS p, TS b) { return fma(b, p, z); }
 
 template <typename T>
 struct Vec2Of;
@@ -274,87 +288,97 @@ struct Vec2Of<float> { using type = float2; };
 template <>
 struct Vec2Of<double> { using type = double2; };
 
-// Loads KS consecutive values (KS even -> vector loads).
-template <typename TS, int KS>
-__device__ __forceinline__ void load_vec(const TS *__restrict__ p, TS (&v)[KS])
+// Loads / stores N consecutive values (N even -> two-wide vector accesses).
+template <typename TS, int N>
+__device__ __forceinline__ void load_vec(const TS *__restrict__ p, TS (&v)[N])
 {
-  if constexpr (KS % 2 == 0) {
+  if constexpr (N % 2 == 0) {
     using V2 = typename Vec2Of<TS>::type;
     const V2 *q = reinterpret_cast<const V2 *>(p);
 #pragma unroll
-    for (int k = 0; k < KS / 2; k++) {
+    for (int k = 0; k < N / 2; k++) {
       const V2 t = q[k];
       v[2 * k] = t.x;
       v[2 * k + 1] = t.y;
     }
   } else {
 #pragma unroll
-    for (int k = 0; k < KS; k++) v[k] = p[k];
+    for (int k = 0; k < N; k++) v[k] = p[k];
   }
 }
-template <typename TS, int KS>
-__device__ __forceinline__ void store_vec(TS *__restrict__ p, const TS (&v)[KS])
+template <typename TS, int N>
+__device__ __forceinline__ void store_vec(TS *__restrict__ p, const TS (&v)[N])
 {
-  if constexpr (KS % 2 == 0) {
+  if constexpr (N % 2 == 0) {
     using V2 = typename Vec2Of<TS>::type;
     V2 *q = reinterpret_cast<V2 *>(p);
 #pragma unroll
-    for (int k = 0; k < KS / 2; k++) q[k] = V2{v[2 * k], v[2 * k + 1]};
+    for (int k = 0; k < N / 2; k++) q[k] = V2{v[2 * k], v[2 * k + 1]};
   } else {
 #pragma unroll
-    for (int k = 0; k < KS; k++) p[k] = v[k];
+    for (int k = 0; k < N; k++) p[k] = v[k];
   }
 }
 
-// One ELL row times p = z + beta p_old for KS right-hand sides.
+// w = A z for the three rows of one node and KS load cases: 27 slot blocks,
+// each one column id, a 3x3 value block and one gathered 3*KS vector run.
 template <typename TS, int KS>
-__device__ __forceinline__ void ell_row_spmv(const int *__restrict__ cols, const TS *__restrict__ vals,
-                                             long long sb, int S, int tid, long long rbase,
-                                             const TS *__restrict__ z, const TS *__restrict__ pold,
-                                             const TS (&beta)[KS], TS (&acc)[KS])
+__device__ __forceinline__ void node_spmv(const int *__restrict__ bcols, const TS *__restrict__ vals,
+                                          long long cb, long long vb, int SN, int tid, long long rbase,
+                                          const TS *__restrict__ z, TS (&w)[3][KS])
 {
+  const int RS = 3 * SN;
 #pragma unroll
-  for (int k = 0; k < KS; k++) acc[k] = (TS)0;
+  for (int d = 0; d < 3; d++)
+#pragma unroll
+    for (int k = 0; k < KS; k++) w[d][k] = (TS)0;
 #pragma unroll 3
-  for (int j = 0; j < kW; j++) {
-    const int c = __builtin_nontemporal_load(&cols[sb + (long long)j * S + tid]);
-    const TS v = __builtin_nontemporal_load(&vals[sb + (long long)j * S + tid]);
-    if (c >= 0) {
-      TS zc[KS], pc[KS];
-      load_vec<TS, KS>(z + (rbase + c) * KS, zc);
-      load_vec<TS, KS>(pold + (rbase + c) * KS, pc);
+  for (int t = 0; t < 27; t++) {
+    const int m = __builtin_nontemporal_load(&bcols[cb + (long long)t * SN + tid]);
+    TS v[3][3];  // [d'][d]
 #pragma unroll
-      for (int k = 0; k < KS; k++) acc[k] = fma(v, pform(zc[k], pc[k], beta[k]), acc[k]);
+    for (int dd = 0; dd < 3; dd++)
+#pragma unroll
+      for (int d = 0; d < 3; d++)
+        v[dd][d] = __builtin_nontemporal_load(&vals[vb + (long long)(3 * t + dd) * RS + 3 * tid + d]);
+    if (m >= 0) {
+      TS xz[3 * KS];
+      load_vec<TS, 3 * KS>(z + (rbase + 3LL * m) * KS, xz);
+#pragma unroll
+      for (int dd = 0; dd < 3; dd++)
+#pragma unroll
+        for (int d = 0; d < 3; d++)
+#pragma unroll
+          for (int k = 0; k < KS; k++) w[d][k] = fma(v[dd][d], xz[dd * KS + k], w[d][k]);
     }
   }
 }
 
-
 // ---------------------------------------------------------------------------
-// Kernel A: PH_INNER (q = A p, alpha) and PH_XUPD (x64 += d).
+// Kernel A: PH_INNER and PH_XUPD.
+// INNER: w = A z, then p = z + beta p and q = w + beta q (so q = A p without
+// gathering p), and p.q; the last workgroup forms alpha = r.z / p.q.
+// XUPD: x64 += d.
 template <typename TS, int K>
-__global__ __launch_bounds__(512) void k_step_a(EngPtrs P, SolveParams sp)
+__global__ __launch_bounds__(256) void k_step_a(EngPtrs P, SolveParams sp)
 {
   constexpr int KS = vec_stride(K);
   __shared__ double lds[K * 16];
-  const int slab = blockIdx.x, tid = threadIdx.x, S = P.S;
+  const int slab = blockIdx.x, tid = threadIdx.x, SN = P.SN;
   const int slot = P.slab_rve[slab];
   if (slot < 0) return;
   RveDev *R = P.rves + slot;
   const int phase = R->phase;
   if (phase != PH_INNER && phase != PH_XUPD) return;
-  const int row = P.slab_row0[slab] + tid;
-  const bool valid = row < R->nrow;
+  const int n = P.slab_node0[slab] + tid;
+  const bool valid = n < R->nrow / 3;
   const long long rbase = R->row_base;
-  const long long grow = rbase + row;
+  const long long grow0 = rbase + 3LL * n;
   const int nslab = R->nslab;
   double *part = P.part + (size_t)slab * kPart;
 
   if (phase == PH_INNER) {
-    const TS *vals = (const TS *)P.vals;
-    const TS *z = (const TS *)P.z;
-    const TS *pold = (const TS *)(R->pbuf ? P.pB : P.pA);
-    TS *pnew = (TS *)(R->pbuf ? P.pA : P.pB);
+    TS *p = (TS *)P.p;
     TS *q = (TS *)P.q;
     TS beta[KS];
 #pragma unroll
@@ -363,47 +387,263 @@ __global__ __launch_bounds__(512) void k_step_a(EngPtrs P, SolveParams sp)
 #pragma unroll
     for (int k = 0; k < K; k++) v[k] = 0.0;
     if (valid) {
-      TS acc[KS];
-      ell_row_spmv<TS, KS>(P.cols, vals, (long long)slab * S * kW, S, tid, rbase, z, pold, beta, acc);
-      TS zi[KS], pi[KS];
-      load_vec<TS, KS>(z + grow * KS, zi);
-      load_vec<TS, KS>(pold + grow * KS, pi);
+      TS w[3][KS];
+      node_spmv<TS, KS>(P.bcols, (const TS *)P.vals, (long long)slab * 27 * SN, (long long)slab * kW * 3 * SN,
+                        SN, tid, rbase, (const TS *)P.z, w);
+      TS zi[3 * KS], pi[3 * KS], qi[3 * KS];
+      load_vec<TS, 3 * KS>((const TS *)P.z + grow0 * KS, zi);
+      load_vec<TS, 3 * KS>(p + grow0 * KS, pi);
+      load_vec<TS, 3 * KS>(q + grow0 * KS, qi);
 #pragma unroll
-      for (int k = 0; k < KS; k++) pi[k] = pform(zi[k], pi[k], beta[k]);
-      store_vec<TS, KS>(pnew + grow * KS, pi);
-      store_vec<TS, KS>(q + grow * KS, acc);
+      for (int d = 0; d < 3; d++)
 #pragma unroll
-      for (int k = 0; k < K; k++) v[k] = (double)pi[k] * (double)acc[k];
+        for (int k = 0; k < KS; k++) {
+          const int i = d * KS + k;
+          pi[i] = fma(beta[k], pi[i], zi[i]);
+          qi[i] = fma(beta[k], qi[i], w[d][k]);
+          if (k < K) v[k < K ? k : 0] += (double)pi[i] * (double)qi[i];
+        }
+      store_vec<TS, 3 * KS>(p + grow0 * KS, pi);
+      store_vec<TS, 3 * KS>(q + grow0 * KS, qi);
     }
     block_sum<K>(v, lds);
     if (tid == 0)
       for (int k = 0; k < K; k++) part[k] = v[k];
-    if (arrive_last(&R->cntA, nslab) && tid == 0) {
-      const double *p0 = P.part + (size_t)R->first_slab * kPart;
-      for (int k = 0; k < K; k++) {
-        double s = 0.0;
-        for (int b = 0; b < nslab; b++) s += load_coherent(&p0[(size_t)b * kPart + k]);
-        const bool done = (R->done_in >> k) & 1u;
-        R->alpha[k] = (done || s == 0.0) ? 0.0 : R->rz[k] / s;
-      }
-      R->pbuf ^= 1;
+    if (arrive_last(&R->cntA, nslab)) {
+      double s[K];
+      group_reduce<K>(P.part + (size_t)R->first_slab * kPart, nslab, kPart, 0, s, lds);
+      if (tid == 0)
+        for (int k = 0; k < K; k++) {
+          const bool done = (R->done_in >> k) & 1u;
+          R->alpha[k] = (done || s[k] == 0.0) ? 0.0 : R->rz[k] / s[k];
+        }
     }
     return;
   }
 
-  if (phase == PH_XUPD) {
-    if (valid) {
-      const TS *d = (const TS *)P.d;
+  // PH_XUPD
+  if (valid) {
+    const TS *d = (const TS *)P.d;
+    for (int r = 0; r < 3; r++)
 #pragma unroll
-      for (int k = 0; k < K; k++) P.x64[grow * KS + k] += (double)d[grow * KS + k];
-    }
-    if (arrive_last(&R->cntA, nslab) && tid == 0) {
-      R->sweep += 1;
-      R->phase = PH_RES;
-    }
-    return;
+      for (int k = 0; k < K; k++) P.x64[(grow0 + r) * KS + k] += (double)d[(grow0 + r) * KS + k];
   }
+  if (arrive_last(&R->cntA, nslab) && tid == 0) {
+    R->sweep += 1;
+    R->phase = PH_RES;
+  }
+}
 
+// ---------------------------------------------------------------------------
+// Kernel B: PH_INNER (d += alpha p, r -= alpha q, z = dinv r, r.r, r.z); the
+// last workgroup checks the per-load-case convergence and forms beta.
+template <typename TS, int K>
+__global__ __launch_bounds__(256) void k_step_b(EngPtrs P, SolveParams sp)
+{
+  constexpr int KS = vec_stride(K);
+  __shared__ double lds[2 * K * 16];
+  const int slab = blockIdx.x, tid = threadIdx.x;
+  const int slot = P.slab_rve[slab];
+  if (slot < 0) return;
+  RveDev *R = P.rves + slot;
+  if (R->phase != PH_INNER) return;
+  const int n = P.slab_node0[slab] + tid;
+  const bool valid = n < R->nrow / 3;
+  const long long grow0 = R->row_base + 3LL * n;
+  const int nslab = R->nslab;
+  double *part = P.part + (size_t)slab * kPart;
+  double v[2 * K];
+#pragma unroll
+  for (int i = 0; i < 2 * K; i++) v[i] = 0.0;
+  if (valid) {
+    TS *dv = (TS *)P.d, *rv = (TS *)P.r, *zv = (TS *)P.z;
+    const TS *p = (const TS *)P.p, *q = (const TS *)P.q, *dinv = (const TS *)P.dinv;
+    TS alpha[KS];
+#pragma unroll
+    for (int k = 0; k < KS; k++) alpha[k] = (k < K) ? (TS)R->alpha[k] : (TS)0;
+    TS di[3 * KS], ri[3 * KS], pi[3 * KS], qi[3 * KS], zi[3 * KS];
+    load_vec<TS, 3 * KS>(dv + grow0 * KS, di);
+    load_vec<TS, 3 * KS>(rv + grow0 * KS, ri);
+    load_vec<TS, 3 * KS>(p + grow0 * KS, pi);
+    load_vec<TS, 3 * KS>(q + grow0 * KS, qi);
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+      const TS dg = dinv[grow0 + d];
+#pragma unroll
+      for (int k = 0; k < KS; k++) {
+        const int i = d * KS + k;
+        di[i] = fma(alpha[k], pi[i], di[i]);
+        ri[i] = fma(-alpha[k], qi[i], ri[i]);
+        zi[i] = dg * ri[i];
+        if (k < K) {
+          v[k < K ? k : 0] += (double)ri[i] * (double)ri[i];
+          v[K + (k < K ? k : 0)] += (double)ri[i] * (double)zi[i];
+        }
+      }
+    }
+    store_vec<TS, 3 * KS>(dv + grow0 * KS, di);
+    store_vec<TS, 3 * KS>(rv + grow0 * KS, ri);
+    store_vec<TS, 3 * KS>(zv + grow0 * KS, zi);
+  }
+  block_sum<2 * K>(v, lds);
+  if (tid == 0)
+    for (int i = 0; i < 2 * K; i++) part[K + i] = v[i];
+  if (!arrive_last(&R->cntB, nslab)) return;
+  double s2[2 * K];
+  group_reduce<2 * K>(P.part + (size_t)R->first_slab * kPart, nslab, kPart, K, s2, lds);
+  if (tid == 0) {
+    R->its += 1;
+    unsigned done = R->done_in;
+    for (int k = 0; k < K; k++) {
+      const double rr = s2[k], rz = s2[K + k];
+      if ((done >> k) & 1u) {
+        R->beta[k] = 0.0;
+        continue;
+      }
+      if (sqrt(rr) <= sp.inner_rtol * R->bnorm_in[k] || R->rz[k] == 0.0) {
+        done |= 1u << k;
+        R->beta[k] = 0.0;
+      } else {
+        R->beta[k] = rz / R->rz[k];
+        R->rz[k] = rz;
+      }
+    }
+    R->done_in = done;
+    const unsigned all = (1u << K) - 1u;
+    if ((done & all) == all || R->its >= sp.max_inner_its) {
+      R->its_total += R->its;
+      R->phase = PH_XUPD;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Residual kernel: PH_RES, the FP64 residual r = -K x64 computed matrix-free
+// from the element stiffness tables (the Newton / refinement residual of
+// micro/src/assembly.c:33-82; boundary rows keep a zero residual, :59-74),
+// then the start of the next PCG sweep (d = 0, z = dinv r, p = q = 0).
+// The relative target refers to the cold-start right-hand side (interior
+// displacement zero, the ||b|| of the first solve); a warm-started RVE
+// measures it at sweep 0 from the boundary neighbours alone.
+template <typename TS, int K>
+__global__ __launch_bounds__(256) void k_step_res(EngPtrs P, SolveParams sp)
+{
+  constexpr int KS = vec_stride(K);
+  __shared__ double lds[4 * K * 16];
+  const int slab = blockIdx.x, tid = threadIdx.x;
+  const int slot = P.slab_rve[slab];
+  if (slot < 0) return;
+  RveDev *R = P.rves + slot;
+  if (R->phase != PH_RES) return;
+  const int n = P.slab_node0[slab] + tid;
+  const bool valid = n < R->nrow / 3;
+  const long long rbase = R->row_base;
+  const int nslab = R->nslab;
+  double *part = P.part + (size_t)slab * kPart;
+  const bool need_cold = R->warm && R->sweep == 0;
+  double v[4 * K];
+#pragma unroll
+  for (int i = 0; i < 4 * K; i++) v[i] = 0.0;
+  if (valid) {
+    TS *dv = (TS *)P.d, *rv = (TS *)P.r, *zv = (TS *)P.z;
+    const TS *dinv = (const TS *)P.dinv;
+    const int nx = R->nx, nxy = R->nx * R->ny;
+    const int ix = n % nx, iy = (n / nx) % R->ny, iz = n / nxy;
+    const int cls = node_class(ix, iy, iz, nx, R->ny, R->nz);
+    const double *ke0 = P.ke_tab + (size_t)slot * kNmat * kKe;
+#pragma unroll 1
+    for (int d = 0; d < 3; d++) {
+      const long long grow = rbase + 3LL * n + d;
+      double r64[K], rc64[K];
+#pragma unroll
+      for (int k = 0; k < K; k++) {
+        r64[k] = 0.0;
+        rc64[k] = 0.0;
+      }
+      if (cls == 13) {
+        const int ne = g_slots.nelem[cls];
+#pragma unroll 1
+        for (int i = 0; i < ne; i++) {
+          const int code = g_slots.elem[cls][i];
+          const int ox = (code & 1) - 1, oy = ((code >> 1) & 1) - 1, oz = ((code >> 2) & 1) - 1;
+          const int a = (code >> 3) & 7;
+          const int e = (ix + ox) + (iy + oy) * R->nex + (iz + oz) * R->nex * R->ney;
+          const double *ke = ke0 + (size_t)P.elem_mat[R->elem_base + e] * kKe + (3 * a + d) * kNd;
+          const int n0 = n + ox + oy * nx + oz * nxy;
+#pragma unroll 2
+          for (int b = 0; b < 8; b++) {
+            int lx, ly, lz;
+            local_coords(b, &lx, &ly, &lz);
+            const long long mrow = rbase + 3LL * (n0 + lx + ly * nx + lz * nxy);
+            const int jx = ix + ox + lx, jy = iy + oy + ly, jz = iz + oz + lz;
+            const bool nb_bc = jx == 0 || jx == nx - 1 || jy == 0 || jy == R->ny - 1 || jz == 0 ||
+                               jz == R->nz - 1;
+            for (int dd = 0; dd < 3; dd++) {
+              const double kv = ke[3 * b + dd];
+              const double *xv = P.x64 + (mrow + dd) * KS;
+#pragma unroll
+              for (int k = 0; k < K; k++) r64[k] = fma(kv, xv[k], r64[k]);
+              if (need_cold && nb_bc)
+#pragma unroll
+                for (int k = 0; k < K; k++) rc64[k] = fma(kv, xv[k], rc64[k]);
+            }
+          }
+        }
+      }
+      TS ri[KS], zi[KS], zero[KS];
+      const TS dg = dinv[grow];
+#pragma unroll
+      for (int k = 0; k < KS; k++) {
+        const int kk = k < K ? k : 0;
+        const double rk = (k < K) ? -r64[kk] : 0.0;
+        ri[k] = (TS)rk;
+        zi[k] = dg * ri[k];
+        zero[k] = (TS)0;
+        if (k < K) {
+          v[kk] += rk * rk;
+          v[K + kk] += (double)ri[k] * (double)ri[k];
+          v[2 * K + kk] += (double)ri[k] * (double)zi[k];
+          v[3 * K + kk] += rc64[kk] * rc64[kk];
+        }
+      }
+      store_vec<TS, KS>(rv + grow * KS, ri);
+      store_vec<TS, KS>(zv + grow * KS, zi);
+      store_vec<TS, KS>(dv + grow * KS, zero);
+      store_vec<TS, KS>((TS *)P.p + grow * KS, zero);
+      store_vec<TS, KS>((TS *)P.q + grow * KS, zero);
+    }
+  }
+  block_sum<4 * K>(v, lds);
+  if (tid == 0)
+    for (int i = 0; i < 4 * K; i++) part[i] = v[i];
+  if (!arrive_last(&R->cntB, nslab)) return;
+  double s4[4 * K];
+  group_reduce<4 * K>(P.part + (size_t)R->first_slab * kPart, nslab, kPart, 0, s4, lds);
+  if (tid == 0) {
+    unsigned done_out = 0;
+    for (int k = 0; k < K; k++) {
+      const double s[4] = {s4[k], s4[K + k], s4[2 * K + k], s4[3 * K + k]};
+      const double rn = sqrt(s[0]);
+      R->rnorm[k] = rn;
+      if (R->sweep == 0) R->r0[k] = need_cold ? sqrt(s[3]) : rn;
+      const bool conv = (k >= R->nload) || R->r0[k] == 0.0 || rn <= sp.rtol * R->r0[k] ||
+                        rn < sp.abs_tol || R->sweep >= sp.max_sweeps;
+      if (conv) done_out |= 1u << k;
+      R->bnorm_in[k] = sqrt(s[1]);
+      R->rz[k] = s[2];
+      R->beta[k] = 0.0;
+      R->alpha[k] = 0.0;
+    }
+    const unsigned all = (1u << K) - 1u;
+    R->done_out = done_out;
+    if ((done_out & all) == all) {
+      R->phase = PH_AVG;
+    } else {
+      R->done_in = done_out;
+      R->its = 0;
+      R->phase = PH_INNER;
+    }
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -411,10 +651,13 @@ __global__ __launch_bounds__(512) void k_step_a(EngPtrs P, SolveParams sp)
 // in FP64, sum_gp (.) w_gp per element through sum_gp B w, then the tangent
 // (micro/src/homogenize.c:80).  One load case at a time keeps registers low.
 template <typename TS, int K>
-__global__ __launch_bounds__(512) void k_step_avg(EngPtrs P, SolveParams sp)
+__global__ __launch_bounds__(256) void k_step_avg(EngPtrs P, SolveParams sp)
 {
   constexpr int KS = vec_stride(K);
   __shared__ double lds[12 * 16];
+  __shared__ double bw[144];
+  __shared__ double cm[kNmat * 36];
+  __shared__ double sig[MM_MAX_RHS][6];
   const int slab = blockIdx.x, tid = threadIdx.x;
   const int slot = P.slab_rve[slab];
   if (slot < 0) return;
@@ -427,9 +670,6 @@ __global__ __launch_bounds__(512) void k_step_avg(EngPtrs P, SolveParams sp)
   const int nelm = R->nelm;
   const int epb = (nelm + nslab - 1) / nslab;
   const int e_end = min(nelm, (ls + 1) * epb);
-  __shared__ double bw[144];
-  __shared__ double cm[kNmat * 36];
-  __shared__ double sig[MM_MAX_RHS][6];
   for (int i = tid; i < 144; i += blockDim.x) bw[i] = P.bw_tab[(size_t)slot * 144 + i];
   for (int i = tid; i < kNmat * 36; i += blockDim.x) cm[i] = P.c_tab[(size_t)slot * kNmat * 36 + i];
   __syncthreads();
@@ -467,8 +707,16 @@ __global__ __launch_bounds__(512) void k_step_avg(EngPtrs P, SolveParams sp)
     if (tid == 0)
       for (int i = 0; i < 12; i++) part[k * 12 + i] = acc[i];
   }
-  if (arrive_last(&R->cntA, nslab) && tid == 0) {
-    const double *p0 = P.part + (size_t)R->first_slab * kPart;
+  if (!arrive_last(&R->cntA, nslab)) return;
+  __shared__ double tot[MM_MAX_RHS][12];
+#pragma unroll 1
+  for (int k = 0; k < K; k++) {
+    double s12[12];
+    group_reduce<12>(P.part + (size_t)R->first_slab * kPart, nslab, kPart, k * 12, s12, lds);
+    if (tid == 0)
+      for (int i = 0; i < 12; i++) tot[k][i] = s12[i];
+  }
+  if (tid == 0) {
     RveOut *o = P.out + slot;
     double rel = 0.0;
     bool ok = true;
@@ -476,9 +724,7 @@ __global__ __launch_bounds__(512) void k_step_avg(EngPtrs P, SolveParams sp)
     for (int k = 0; k < K; k++) {
 #pragma unroll 1
       for (int i = 0; i < 12; i++) {
-        double s = 0.0;
-        for (int b = 0; b < nslab; b++) s += load_coherent(&p0[(size_t)b * kPart + k * 12 + i]);
-        s = s / R->vol;
+        const double s = tot[k][i] / R->vol;
         if (i < 6) o->strain[k][i] = s;
         else {
           o->stress[k][i - 6] = s;
@@ -512,264 +758,42 @@ __global__ __launch_bounds__(512) void k_step_avg(EngPtrs P, SolveParams sp)
   }
 }
 
-// ---------------------------------------------------------------------------
-// Kernel B: PH_INNER (d += alpha p, r -= alpha q, z = dinv r, r.r, r.z).
+// Stand-alone copy of the INNER-phase SpMV (w = A z) over every resident RVE,
+// written to q; used only when no RVE is being solved.
 template <typename TS, int K>
-__global__ __launch_bounds__(512) void k_step_b(EngPtrs P, SolveParams sp)
+__global__ __launch_bounds__(256) void k_spmv_time(EngPtrs P)
 {
   constexpr int KS = vec_stride(K);
-  __shared__ double lds[2 * K * 16];
-  const int slab = blockIdx.x, tid = threadIdx.x;
-  const int slot = P.slab_rve[slab];
-  if (slot < 0) return;
-  RveDev *R = P.rves + slot;
-  const int phase = R->phase;
-  if (phase != PH_INNER) return;
-  const int row = P.slab_row0[slab] + tid;
-  const bool valid = row < R->nrow;
-  const long long rbase = R->row_base;
-  const long long grow = rbase + row;
-  const int nslab = R->nslab;
-  double *part = P.part + (size_t)slab * kPart;
-  TS *dv = (TS *)P.d;
-  TS *rv = (TS *)P.r;
-  TS *zv = (TS *)P.z;
-  const TS *dinv = (const TS *)P.dinv;
-
-  {
-    const TS *p = (const TS *)(R->pbuf ? P.pB : P.pA);  // written by kernel A
-    const TS *q = (const TS *)P.q;
-    double v[2 * K];
-#pragma unroll
-    for (int i = 0; i < 2 * K; i++) v[i] = 0.0;
-    if (valid) {
-      TS alpha[KS];
-#pragma unroll
-      for (int k = 0; k < KS; k++) alpha[k] = (k < K) ? (TS)R->alpha[k] : (TS)0;
-      TS di[KS], ri[KS], pi[KS], qi[KS], zi[KS];
-      load_vec<TS, KS>(dv + grow * KS, di);
-      load_vec<TS, KS>(rv + grow * KS, ri);
-      load_vec<TS, KS>(p + grow * KS, pi);
-      load_vec<TS, KS>(q + grow * KS, qi);
-      const TS dg = dinv[grow];
-#pragma unroll
-      for (int k = 0; k < KS; k++) {
-        di[k] = fma(alpha[k], pi[k], di[k]);
-        ri[k] = fma(-alpha[k], qi[k], ri[k]);
-        zi[k] = dg * ri[k];
-      }
-      store_vec<TS, KS>(dv + grow * KS, di);
-      store_vec<TS, KS>(rv + grow * KS, ri);
-      store_vec<TS, KS>(zv + grow * KS, zi);
-#pragma unroll
-      for (int k = 0; k < K; k++) {
-        v[k] = (double)ri[k] * (double)ri[k];
-        v[K + k] = (double)ri[k] * (double)zi[k];
-      }
-    }
-    block_sum<2 * K>(v, lds);
-    if (tid == 0)
-      for (int i = 0; i < 2 * K; i++) part[K + i] = v[i];
-    if (arrive_last(&R->cntB, nslab) && tid == 0) {
-      const double *p0 = P.part + (size_t)R->first_slab * kPart;
-      R->its += 1;
-      unsigned done = R->done_in;
-      for (int k = 0; k < K; k++) {
-        double rr = 0.0, rz = 0.0;
-        for (int b = 0; b < nslab; b++) {
-          rr += load_coherent(&p0[(size_t)b * kPart + K + k]);
-          rz += load_coherent(&p0[(size_t)b * kPart + 2 * K + k]);
-        }
-        if ((done >> k) & 1u) {
-          R->beta[k] = 0.0;
-          continue;
-        }
-        if (sqrt(rr) <= sp.inner_rtol * R->bnorm_in[k] || R->rz[k] == 0.0) {
-          done |= 1u << k;
-          R->beta[k] = 0.0;
-        } else {
-          R->beta[k] = rz / R->rz[k];
-          R->rz[k] = rz;
-        }
-      }
-      R->done_in = done;
-      const unsigned all = (1u << K) - 1u;
-      if ((done & all) == all || R->its >= sp.max_inner_its) {
-        R->its_total += R->its;
-        R->phase = PH_XUPD;
-      }
-    }
-    return;
-  }
-
-}
-
-// ---------------------------------------------------------------------------
-// Residual kernel: PH_RES, the FP64 residual r = -K x64 computed matrix-free
-// from the element stiffness tables (the Newton / refinement residual of
-// micro/src/assembly.c:33-82), then the start of the next PCG sweep.
-template <typename TS, int K>
-__global__ __launch_bounds__(512) void k_step_res(EngPtrs P, SolveParams sp)
-{
-  constexpr int KS = vec_stride(K);
-  __shared__ double lds[4 * K * 16];
-  const int slab = blockIdx.x, tid = threadIdx.x;
-  const int slot = P.slab_rve[slab];
-  if (slot < 0) return;
-  RveDev *R = P.rves + slot;
-  if (R->phase != PH_RES) return;
-  const int row = P.slab_row0[slab] + tid;
-  const bool valid = row < R->nrow;
-  const long long rbase = R->row_base;
-  const long long grow = rbase + row;
-  const int nslab = R->nslab;
-  double *part = P.part + (size_t)slab * kPart;
-  TS *dv = (TS *)P.d;
-  TS *rv = (TS *)P.r;
-  TS *zv = (TS *)P.z;
-  const TS *dinv = (const TS *)P.dinv;
-
-  // PH_RES.  The relative target refers to the cold-start right-hand side
-  // (interior displacement zero, the ||b|| of the first solve); a warm-started
-  // RVE measures it at sweep 0 from the boundary neighbours alone.
-  const bool need_cold = R->warm && R->sweep == 0;
-  double v[4 * K];
-#pragma unroll
-  for (int i = 0; i < 4 * K; i++) v[i] = 0.0;
-  if (valid) {
-    const int n = row / 3, d = row - 3 * (row / 3);
-    const int nx = R->nx, nxy = R->nx * R->ny;
-    const int ix = n % nx, iy = (n / nx) % R->ny, iz = n / nxy;
-    const int cx = (ix == 0) ? 0 : ((ix == nx - 1) ? 2 : 1);
-    const int cy = (iy == 0) ? 0 : ((iy == R->ny - 1) ? 2 : 1);
-    const int cz = (iz == 0) ? 0 : ((iz == R->nz - 1) ? 2 : 1);
-    const int cls = cx + 3 * cy + 9 * cz;
-    double r64[K], rc64[K];
-#pragma unroll
-    for (int k = 0; k < K; k++) {
-      r64[k] = 0.0;
-      rc64[k] = 0.0;
-    }
-    if (cls == 13) {  // boundary rows keep a zero residual (micro/src/assembly.c:59-74)
-      const double *ke0 = P.ke_tab + (size_t)slot * kNmat * kKe;
-      const int ne = g_slots.nelem[cls];
-#pragma unroll 1
-      for (int i = 0; i < ne; i++) {
-        const int code = g_slots.elem[cls][i];
-        const int ox = (code & 1) - 1, oy = ((code >> 1) & 1) - 1, oz = ((code >> 2) & 1) - 1;
-        const int a = (code >> 3) & 7;
-        const int e = (ix + ox) + (iy + oy) * R->nex + (iz + oz) * R->nex * R->ney;
-        const double *ke = ke0 + (size_t)P.elem_mat[R->elem_base + e] * kKe + (3 * a + d) * kNd;
-        const int n0 = n + ox + oy * nx + oz * nxy;
-#pragma unroll 2
-        for (int b = 0; b < 8; b++) {
-          int lx, ly, lz;
-          local_coords(b, &lx, &ly, &lz);
-          const long long mrow = rbase + 3LL * (n0 + lx + ly * nx + lz * nxy);
-          const int jx = ix + ox + lx, jy = iy + oy + ly, jz = iz + oz + lz;
-          const bool nb_bc = jx == 0 || jx == nx - 1 || jy == 0 || jy == R->ny - 1 || jz == 0 ||
-                             jz == R->nz - 1;
-          for (int dd = 0; dd < 3; dd++) {
-            const double kv = ke[3 * b + dd];
-            const double *xv = P.x64 + (mrow + dd) * KS;
-#pragma unroll
-            for (int k = 0; k < K; k++) r64[k] = fma(kv, xv[k], r64[k]);
-            if (need_cold && nb_bc)
-#pragma unroll
-              for (int k = 0; k < K; k++) rc64[k] = fma(kv, xv[k], rc64[k]);
-          }
-        }
-      }
-    }
-    TS ri[KS], zi[KS], zero[KS];
-    const TS dg = dinv[grow];
-#pragma unroll
-    for (int k = 0; k < KS; k++) {
-      const double rk = (k < K) ? -r64[k < K ? k : 0] : 0.0;
-      ri[k] = (TS)rk;
-      zi[k] = dg * ri[k];
-      zero[k] = (TS)0;
-      if (k < K) {
-        v[k] = rk * rk;
-        v[K + k] = (double)ri[k] * (double)ri[k];
-        v[2 * K + k] = (double)ri[k] * (double)zi[k];
-        v[3 * K + k] = rc64[k < K ? k : 0] * rc64[k < K ? k : 0];
-      }
-    }
-    store_vec<TS, KS>(rv + grow * KS, ri);
-    store_vec<TS, KS>(zv + grow * KS, zi);
-    store_vec<TS, KS>(dv + grow * KS, zero);
-    store_vec<TS, KS>((TS *)P.pA + grow * KS, zero);
-    store_vec<TS, KS>((TS *)P.pB + grow * KS, zero);
-  }
-  block_sum<4 * K>(v, lds);
-  if (tid == 0)
-    for (int i = 0; i < 4 * K; i++) part[i] = v[i];
-  if (arrive_last(&R->cntB, nslab) && tid == 0) {
-    const double *p0 = P.part + (size_t)R->first_slab * kPart;
-    unsigned done_out = 0;
-    for (int k = 0; k < K; k++) {
-      double s[4] = {0.0, 0.0, 0.0, 0.0};
-      for (int b = 0; b < nslab; b++)
-        for (int i = 0; i < 4; i++) s[i] += load_coherent(&p0[(size_t)b * kPart + i * K + k]);
-      const double rn = sqrt(s[0]);
-      R->rnorm[k] = rn;
-      if (R->sweep == 0) R->r0[k] = need_cold ? sqrt(s[3]) : rn;
-      const bool conv = (k >= R->nload) || R->r0[k] == 0.0 || rn <= sp.rtol * R->r0[k] ||
-                        rn < sp.abs_tol || R->sweep >= sp.max_sweeps;
-      if (conv) done_out |= 1u << k;
-      R->bnorm_in[k] = sqrt(s[1]);
-      R->rz[k] = s[2];
-      R->beta[k] = 0.0;
-      R->alpha[k] = 0.0;
-    }
-    const unsigned all = (1u << K) - 1u;
-    R->done_out = done_out;
-    if ((done_out & all) == all) {
-      R->phase = PH_AVG;
-    } else {
-      R->done_in = done_out;
-      R->its = 0;
-      R->pbuf = 0;
-      R->phase = PH_INNER;
-    }
-  }
-}
-
-// Stand-alone timing copy of the INNER-phase SpMV for every resident RVE:
-// q = A (z + beta p_old), no solver state change.
-template <typename TS, int K>
-__global__ __launch_bounds__(512) void k_spmv_time(EngPtrs P)
-{
-  constexpr int KS = vec_stride(K);
-  const int slab = blockIdx.x, tid = threadIdx.x, S = P.S;
+  const int slab = blockIdx.x, tid = threadIdx.x, SN = P.SN;
   const int slot = P.slab_rve[slab];
   if (slot < 0) return;
   const RveDev *R = P.rves + slot;
-  const int row = P.slab_row0[slab] + tid;
-  if (row >= R->nrow) return;
+  const int n = P.slab_node0[slab] + tid;
+  if (n >= R->nrow / 3) return;
   const long long rbase = R->row_base;
-  TS beta[KS];
+  TS w[3][KS];
+  node_spmv<TS, KS>(P.bcols, (const TS *)P.vals, (long long)slab * 27 * SN, (long long)slab * kW * 3 * SN,
+                    SN, tid, rbase, (const TS *)P.z, w);
+  TS o[3 * KS];
 #pragma unroll
-  for (int k = 0; k < KS; k++) beta[k] = (TS)0.5;
-  TS acc[KS];
-  ell_row_spmv<TS, KS>(P.cols, (const TS *)P.vals, (long long)slab * S * kW, S, tid, rbase,
-                       (const TS *)P.z, (const TS *)P.pA, beta, acc);
-  store_vec<TS, KS>((TS *)P.q + (rbase + row) * KS, acc);
+  for (int d = 0; d < 3; d++)
+#pragma unroll
+    for (int k = 0; k < KS; k++) o[d * KS + k] = w[d][k];
+  store_vec<TS, 3 * KS>((TS *)P.q + (rbase + 3LL * n) * KS, o);
 }
 
 // ---------------------------------------------------------------------------
-// launchers (explicit instantiations for the supported RHS counts)
+// launchers (explicit instantiations for the supported load-case counts)
 
 template <typename TS, int K>
 static int launch_steps(const EngPtrs &P, const SolveParams &sp, int grid, int nsteps, hipStream_t s)
 {
   for (int i = 0; i < nsteps; i++) {
-    k_step_a<TS, K><<<grid, P.S, 0, s>>>(P, sp);
-    k_step_b<TS, K><<<grid, P.S, 0, s>>>(P, sp);
-    k_step_res<TS, K><<<grid, P.S, 0, s>>>(P, sp);
+    k_step_a<TS, K><<<grid, P.SN, 0, s>>>(P, sp);
+    k_step_b<TS, K><<<grid, P.SN, 0, s>>>(P, sp);
+    k_step_res<TS, K><<<grid, P.SN, 0, s>>>(P, sp);
   }
-  k_step_avg<TS, K><<<grid, P.S, 0, s>>>(P, sp);
+  k_step_avg<TS, K><<<grid, P.SN, 0, s>>>(P, sp);
   MM_HIP(hipGetLastError());
   return 0;
 }
@@ -778,9 +802,9 @@ template <typename TS, int K>
 static int launch_assemble(const EngPtrs &P, int nblk, const int *slab_list, int mode, hipStream_t s)
 {
   if (nblk <= 0) return 0;
-  k_assemble<TS, K><<<nblk, P.S, 0, s>>>(nblk, slab_list, P.slab_rve, P.slab_row0, P.rves, P.ke_tab,
-                                         P.elem_mat, const_cast<int *>(P.cols), (TS *)P.vals,
-                                         (TS *)P.dinv, P.x64, P.S, mode);
+  k_assemble<TS, K><<<nblk, P.SN, 0, s>>>(nblk, slab_list, P.slab_rve, P.slab_node0, P.rves, P.ke_tab,
+                                          P.elem_mat, const_cast<int *>(P.bcols), (TS *)P.vals,
+                                          (TS *)P.dinv, P.x64, P.SN, mode);
   MM_HIP(hipGetLastError());
   return 0;
 }
@@ -788,7 +812,7 @@ static int launch_assemble(const EngPtrs &P, int nblk, const int *slab_list, int
 template <typename TS, int K>
 static int launch_time(const EngPtrs &P, int grid, hipStream_t s)
 {
-  k_spmv_time<TS, K><<<grid, P.S, 0, s>>>(P);
+  k_spmv_time<TS, K><<<grid, P.SN, 0, s>>>(P);
   MM_HIP(hipGetLastError());
   return 0;
 }
@@ -838,10 +862,10 @@ int eng_launch_time(bool fp64, int K, const EngPtrs &P, int grid, hipStream_t s)
 }
 
 int eng_launch_admit(int n, const int *slots, const RveDev *states, RveDev *rves, int *slab_rve,
-                     int *slab_row0, int S, hipStream_t s)
+                     int *slab_node0, int SN, hipStream_t s)
 {
   if (n <= 0) return 0;
-  k_admit<<<n, 256, 0, s>>>(n, slots, states, rves, slab_rve, slab_row0, S);
+  k_admit<<<n, 256, 0, s>>>(n, slots, states, rves, slab_rve, slab_node0, SN);
   MM_HIP(hipGetLastError());
   return 0;
 }

@@ -97,7 +97,7 @@ extern "C" int micro_init(const micro_config *cfg)
   mm_engine_config_default(&ec);
   ec.device = cfg->device;
   ec.max_rves = 2;
-  ec.slab_rows = 256;
+  ec.slab_nodes = 128;
   ec.nrhs = 7;
   ec.use_graph = 1;
   ec.steps_per_launch = 32;
@@ -118,7 +118,7 @@ extern "C" int micro_init(const micro_config *cfg)
     ec.max_sweeps = cfg->nl_max_its > 12 ? cfg->nl_max_its : 12;
   }
   const double rows = 3.0 * (cfg->rve.nex + 1.0) * (cfg->rve.ney + 1.0) * (cfg->rve.nez + 1.0);
-  ec.hbm_budget = (size_t)((rows + 4.0 * ec.slab_rows) * 1800.0) + (size_t)(8u << 20);
+  ec.hbm_budget = (size_t)((rows + 6.0 * ec.slab_nodes) * 1800.0) + (size_t)(8u << 20);
   int rc = mm_engine_create(&ec, &g.eng);
   if (rc) return rc;
   g.init = 1;

@@ -61,4 +61,20 @@ __device__ inline bool arrive_last(int *counter, int nblocks)
   return s_last != 0;
 }
 
+// Run by every thread of the last workgroup of a group: sums NV consecutive
+// partials (offset off) over the group's nslab partial rows (row stride
+// `stride`) in a fixed order (lane-strided, then block_sum).  The total is
+// valid in thread 0.
+template <int NV>
+__device__ inline void group_reduce(const double *p0, int nslab, int stride, int off, double (&out)[NV],
+                                    double *lds)
+{
+#pragma unroll
+  for (int i = 0; i < NV; i++) out[i] = 0.0;
+  for (int b = threadIdx.x; b < nslab; b += blockDim.x)
+#pragma unroll
+    for (int i = 0; i < NV; i++) out[i] += load_coherent(&p0[(size_t)b * stride + off + i]);
+  block_sum<NV>(out, lds);
+}
+
 }  // namespace mm

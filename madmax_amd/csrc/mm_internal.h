@@ -72,7 +72,7 @@ struct RveDev {
   // geometry and materials (written at admission)
   int nex, ney, nez, nx, ny, nz;
   int nrow, nelm, nslab, first_slab;
-  long long row_base;   // global row of local row 0 (first_slab * slab_rows)
+  long long row_base;   // global row of local row 0 (first_slab * 3 * SN)
   long long elem_base;  // offset into the element-material arena
   double hx, hy, hz, vol;
   int nfib, tangent_mode, nload, warm;
@@ -81,7 +81,7 @@ struct RveDev {
   double load[MM_MAX_RHS][MM_NVOI];
   // solver state
   int phase, sweep, its, its_total;
-  int cntA, cntB, pbuf, status;
+  int cntA, cntB, status, pad_;
   unsigned done_in, done_out;
   double alpha[MM_MAX_RHS], beta[MM_MAX_RHS], rz[MM_MAX_RHS];
   double bnorm_in[MM_MAX_RHS], r0[MM_MAX_RHS], rnorm[MM_MAX_RHS];
@@ -98,25 +98,25 @@ struct RveOut {
 };
 
 struct SolveParams {
-  int nrhs, slab_rows;
+  int nrhs, slab_nodes;
   double rtol, abs_tol, inner_rtol;
   int max_inner_its, max_sweeps;
 };
 
 // Device pointers of the engine arena handed to every solver kernel.
 struct EngPtrs {
-  const int *cols;
-  const void *vals;
+  const int *bcols;  // [slab][27][SN] neighbour node of each slot block
+  const void *vals;  // [slab][81][3*SN]
   const void *dinv;
   double *x64;
-  void *d, *r, *z, *pA, *pB, *q;
+  void *d, *r, *z, *p, *q;
   double *part;
-  const int *slab_rve, *slab_row0;
+  const int *slab_rve, *slab_node0;
   RveDev *rves;
   RveOut *out;
   const double *ke_tab, *c_tab, *bw_tab;
   const uint8_t *elem_mat;
-  int S;
+  int SN;  // nodes per slab = lanes per workgroup
 };
 
 // launchers (engine_kernels.hip)
@@ -128,7 +128,7 @@ int eng_launch_assemble(bool fp64, int K, const EngPtrs &P, int nblk, const int 
                         hipStream_t s);
 int eng_launch_time(bool fp64, int K, const EngPtrs &P, int grid, hipStream_t s);
 int eng_launch_admit(int n, const int *slots, const RveDev *states, RveDev *rves, int *slab_rve,
-                     int *slab_row0, int S, hipStream_t s);
+                     int *slab_node0, int SN, hipStream_t s);
 int eng_launch_ke(int n, const int *work, const RveDev *rves, double *ke_tab, double *c_tab_out,
                   double *bw_tab, hipStream_t s);
 int eng_launch_phase(int nblk, const int2 *work, const RveDev *rves, uint8_t *elem_mat, hipStream_t s);

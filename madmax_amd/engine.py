@@ -83,7 +83,7 @@ class Engine:
     """Batched / serving RVE engine (mm_engine_*)."""
 
     def __init__(self, nrhs=6, precision=_lib.MM_PREC_MIXED, hbm_budget=0, max_rves=0,
-                 slab_rows=0, rtol=1e-10, abs_tol=0.0, inner_rtol=1e-5, max_inner_its=20000,
+                 slab_nodes=0, rtol=1e-10, abs_tol=0.0, inner_rtol=1e-5, max_inner_its=20000,
                  max_sweeps=8, steps_per_launch=16, use_graph=True, sync_every_step=False,
                  device=0):
         L = lib()
@@ -92,7 +92,7 @@ class Engine:
         c.device = device
         c.hbm_budget = int(hbm_budget)
         c.max_rves = int(max_rves)
-        c.slab_rows = int(slab_rows)
+        c.slab_nodes = int(slab_nodes)
         c.nrhs = int(nrhs)
         c.precision = int(precision)
         c.rtol = rtol
