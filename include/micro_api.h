@@ -66,6 +66,7 @@ typedef struct {
   double rtol;         /* PCG relative residual target (default 1e-10) */
   int nl_max_its;      /* Newton / refinement sweeps (micro/src/init.c:7, default 2) */
   double nl_min_norm;  /* absolute residual threshold (micro/src/init.c:8, default 1e-4) */
+  double rho;          /* density answered to ACTION_MICRO_CALC_RHO (micro/src/init.c:9) */
   int device;
 } micro_config;
 
@@ -87,6 +88,25 @@ int homog_get_c_tangent(double *strain_mac, double **c_tangent);
 int homog_calculate_c_tangent(double *strain_mac, double *c_tangent);
 /* micro/inc/micro.h:153 */
 int homog_calculate_c_tangent_around_zero(double *c_tangent);
+
+/* The request / response record of the macro-micro coupling, byte-for-byte
+ * the layout of message_t (inc/comm.h:18-27), and the dispatch of one request
+ * as the micro's loop does it (micro/src/main.c:116-151).  Returns 0, or 1
+ * for an unknown action. */
+#define ACTION_NULL 0
+#define ACTION_MICRO_CALC_STRESS 1
+#define ACTION_MICRO_CALC_C_TANGENT 2
+#define ACTION_MICRO_CALC_RHO 3
+#define ACTION_MICRO_END 4
+typedef struct {
+  int action;
+  int num_voigt;
+  double strain_mac[6];
+  double stress_ave[6];
+  double c_tangent_ave[36];
+  double rho;
+} micro_message;
+int micro_serve_message(micro_message *msg);
 
 /* Inspection of the process-global micro (no reference counterpart). */
 typedef struct {

@@ -48,7 +48,19 @@ class RveSpec(C.Structure):
 
 class MicroConfig(C.Structure):
     _fields_ = [("rve", RveSpec), ("solver", C.c_int), ("rtol", C.c_double),
-                ("nl_max_its", C.c_int), ("nl_min_norm", C.c_double), ("device", C.c_int)]
+                ("nl_max_its", C.c_int), ("nl_min_norm", C.c_double), ("rho", C.c_double),
+                ("device", C.c_int)]
+
+
+class MicroMessage(C.Structure):
+    """message_t of inc/comm.h:18-27 (400 bytes)"""
+    _fields_ = [("action", C.c_int), ("num_voigt", C.c_int), ("strain_mac", C.c_double * 6),
+                ("stress_ave", C.c_double * 6), ("c_tangent_ave", C.c_double * 36),
+                ("rho", C.c_double)]
+
+
+ACTION_NULL, ACTION_MICRO_CALC_STRESS, ACTION_MICRO_CALC_C_TANGENT = 0, 1, 2
+ACTION_MICRO_CALC_RHO, ACTION_MICRO_END = 3, 4
 
 
 class MicroStats(C.Structure):
@@ -119,6 +131,7 @@ SIGNATURES = {
     "homog_get_c_tangent": (C.c_int, [dp, C.POINTER(dp)]),
     "homog_calculate_c_tangent": (C.c_int, [dp, dp]),
     "homog_calculate_c_tangent_around_zero": (C.c_int, [dp]),
+    "micro_serve_message": (C.c_int, [C.POINTER(MicroMessage)]),
     "micro_get_stats": (C.c_int, [C.POINTER(MicroStats)]),
     "micro_get_sizes": (C.c_int, [ip, ip, ip]),
     "micro_get_ell": (C.c_int, [ip, dp]),

@@ -72,6 +72,7 @@ extern "C" int micro_config_default(micro_config *c)
   c->rtol = 1.0e-10;
   c->nl_max_its = 2;     // micro/src/init.c:7
   c->nl_min_norm = 1.0e-4;  // micro/src/init.c:8
+  c->rho = 1.0e7;           // micro/src/init.c:9
   c->device = 0;
   return 0;
 }
@@ -218,6 +219,33 @@ extern "C" int homog_get_c_tangent(double *strain_mac, double **c_tangent)
   int rc = homog_calculate_c_tangent(strain_mac, g.c_tangent);
   *c_tangent = g.c_tangent;
   return rc;
+}
+
+// micro/src/main.c:116-151: one coupling request
+extern "C" int micro_serve_message(micro_message *msg)
+{
+  if (!msg) return MM_ERR_ARG;
+  switch (msg->action) {
+    case ACTION_MICRO_CALC_STRESS: {
+      double strain_ave[MM_NVOI];
+      return homog_get_strain_stress(msg->strain_mac, strain_ave, msg->stress_ave);
+    }
+    case ACTION_MICRO_CALC_C_TANGENT: {
+      double *c = nullptr;
+      int rc = homog_get_c_tangent(msg->strain_mac, &c);
+      if (rc) return rc;
+      for (int i = 0; i < MM_NVOI * MM_NVOI; i++) msg->c_tangent_ave[i] = c[i];
+      return 0;
+    }
+    case ACTION_MICRO_CALC_RHO:
+      if (!g.init) return MM_ERR_STATE;
+      msg->rho = g.cfg.rho;
+      return 0;
+    case ACTION_MICRO_END:
+      return 0;
+    default:
+      return 1;
+  }
 }
 
 extern "C" int micro_get_stats(micro_stats *st)
