@@ -49,8 +49,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--batch", type=int, default=128)
     ap.add_argument("--n", type=int, default=30)
-    ap.add_argument("--slab", type=int, default=128, help="nodes per ELL slab (64, 128, 256)")
-    ap.add_argument("--inner-rtol", type=float, default=1e-5)
+    ap.add_argument("--slab", type=int, default=0, help="nodes per ELL slab (64, 128, 256); 0 = autotune")
+    ap.add_argument("--inner-rtol", type=float, default=0.0, help="FP32 PCG target per sweep; 0 = autotune")
+    ap.add_argument("--rtol", type=float, default=1e-9, help="FP64 residual target (relative)")
+    ap.add_argument("--tune-batch", type=int, default=16)
     ap.add_argument("--spl", type=int, default=16, help="PCG steps per graph launch")
     ap.add_argument("--cpu-sample", type=int, default=2, help="RVE tangents timed on the CPU")
     ap.add_argument("--no-plain", action="store_true")
@@ -67,12 +69,23 @@ def main():
 
     B, n = args.batch, args.n
     specs = [configs.c2_spec(2, i, n=n) for i in range(B)]
+    tuned = None
+    if args.slab == 0 or args.inner_rtol == 0.0:
+        from madmax_amd import autotune
+        t = time.perf_counter()
+        tuned = autotune.tune(specs[: args.tune_batch], rtol=args.rtol, log=log, use_cache=False)
+        log("autotune (%.1f s): slab_nodes=%d inner_rtol=%g" % (time.perf_counter() - t,
+                                                                tuned["slab_nodes"], tuned["inner_rtol"]))
+        if args.slab == 0:
+            args.slab = tuned["slab_nodes"]
+        if args.inner_rtol == 0.0:
+            args.inner_rtol = tuned["inner_rtol"]
     rows = 3 * (n + 1) ** 3
     per_rve = rows * 900 + (1 << 20)
     budget = int(per_rve * B * 1.3) + (256 << 20)
     log("batch %d x %d^3 (%d rows each), arena %.1f GB" % (B, n, rows, budget / 1e9))
     eng = mm.Engine(nrhs=6, precision=mm.MM_PREC_MIXED, hbm_budget=budget, max_rves=B + 8,
-                    slab_nodes=args.slab, inner_rtol=args.inner_rtol, rtol=1e-10,
+                    slab_nodes=args.slab, inner_rtol=args.inner_rtol, rtol=args.rtol,
                     steps_per_launch=args.spl, use_graph=True)
     for w in range(args.warmup):
         t = time.perf_counter()
@@ -148,7 +161,8 @@ def main():
     if args.serve > 0:
         stream = configs.c5_stream(count=args.serve, rate=4000.0)
         se = mm.Engine(nrhs=7, precision=mm.MM_PREC_MIXED, hbm_budget=8 << 30, max_rves=512,
-                       slab_nodes=args.slab, inner_rtol=args.inner_rtol, steps_per_launch=8)
+                       slab_nodes=args.slab, inner_rtol=args.inner_rtol, rtol=args.rtol,
+                       steps_per_launch=8)
         se.start()
         t0s = time.perf_counter()
         got = []
@@ -185,7 +199,8 @@ def main():
         "data": "synthetic (seeded two-phase fiber RVEs)",
         "config": {"workload": "configs[1]: %d x %d^3 hex8 RVE tangents per step, 6-RHS block, "
                                "FP32 PCG + FP64 IR" % (B, n), "rve_elements": n ** 3,
-                   "batch": B, "slab_nodes": args.slab, "parallelism": "single GPU"},
+                   "batch": B, "slab_nodes": args.slab, "inner_rtol": args.inner_rtol,
+                   "rtol": args.rtol, "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
                      "kernel": "k_spmv_time<float,6> (batched ELL SpMV, %d resident RVEs)" % B,

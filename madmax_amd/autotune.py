@@ -1,0 +1,96 @@
+"""Autotuner of the batched solver: ELL slab shape (nodes per slab = lanes per
+workgroup) and the per-sweep FP32 PCG tolerance of the mixed-precision path.
+
+For each candidate it solves a sample batch on the GPU and times the whole
+solve (the quantity the engine serves); the SpMV kernel alone is timed with
+HIP events for the record.  Every candidate must converge to the outer FP64
+tolerance, so the choice only trades speed.  Results are cached as JSON keyed
+by device, RVE size, load cases and precision.
+"""
+import json
+import os
+import time
+
+from . import _lib
+from .engine import Engine, unit_loads
+
+SLAB_NODES = (64, 128, 256)
+INNER_RTOLS = (1e-2, 1e-3, 1e-4)
+BASE_INNER = 1e-3
+
+
+def _cache_path():
+    base = os.environ.get("MADMAX_AMD_CACHE", os.path.join(os.path.expanduser("~"), ".cache",
+                                                            "madmax_amd"))
+    os.makedirs(base, exist_ok=True)
+    return os.path.join(base, "tune.json")
+
+
+def _solve_time(specs, slab_nodes, inner_rtol, rtol, nrhs, precision, budget):
+    with Engine(nrhs=nrhs, precision=precision, hbm_budget=budget, max_rves=len(specs) + 4,
+                slab_nodes=slab_nodes, inner_rtol=inner_rtol, rtol=rtol) as eng:
+        def run(keyed):
+            for i, s in enumerate(specs):
+                eng.submit(i + 1, s, unit_loads(), _lib.MM_TANGENT_UNIT,
+                           warm_key=(i + 1) if keyed else 0)
+            eng.drain()
+            return eng.poll(len(specs) + 4)
+        run(False)  # warm-up (graph capture, first-touch allocations)
+        t = time.perf_counter()
+        res = run(True)
+        dt = time.perf_counter() - t
+        ok = all(r["status"] == 0 for r in res)
+        its = max(r["pcg_its"] for r in res)
+        spmv_ms, spmv_bytes = eng.time_spmv(reps=10)
+    return dict(seconds=dt, ok=ok, its=its, spmv_gbs=spmv_bytes / (spmv_ms * 1e-3) / 1e9)
+
+
+def tune(specs, nrhs=6, precision=_lib.MM_PREC_MIXED, rtol=1e-9, budget=None, use_cache=True,
+         log=None):
+    """Returns {"slab_nodes", "inner_rtol", "trials": [...]} for this workload."""
+    n = specs[0].nex
+    key = "%s|n%d|k%d|p%d|b%d|r%g" % (os.environ.get("HIP_VISIBLE_DEVICES", "0"), n, nrhs,
+                                       precision, len(specs), rtol)
+    path = _cache_path()
+    cache = {}
+    if use_cache and os.path.exists(path):
+        try:
+            cache = json.load(open(path))
+        except ValueError:
+            cache = {}
+        if key in cache:
+            return cache[key]
+    rows = 3 * (n + 1) ** 3
+    if budget is None:
+        budget = int(rows * 900 * len(specs) * 1.3) + (256 << 20)
+    trials = []
+    best = None
+    for sn in SLAB_NODES:
+        r = _solve_time(specs, sn, BASE_INNER, rtol, nrhs, precision, budget)
+        r.update(slab_nodes=sn, inner_rtol=BASE_INNER)
+        trials.append(r)
+        if log:
+            log("tune slab_nodes=%d: %.3f s, %d its, spmv %.0f GB/s" % (sn, r["seconds"], r["its"],
+                                                                        r["spmv_gbs"]))
+        if r["ok"] and (best is None or r["seconds"] < best["seconds"]):
+            best = r
+    sn = best["slab_nodes"] if best else 128
+    if precision == _lib.MM_PREC_MIXED:
+        for tol in INNER_RTOLS:
+            if tol == BASE_INNER:
+                continue
+            r = _solve_time(specs, sn, tol, rtol, nrhs, precision, budget)
+            r.update(slab_nodes=sn, inner_rtol=tol)
+            trials.append(r)
+            if log:
+                log("tune inner_rtol=%g: %.3f s, %d its" % (tol, r["seconds"], r["its"]))
+            if r["ok"] and r["seconds"] < best["seconds"]:
+                best = r
+    out = dict(slab_nodes=best["slab_nodes"] if best else 128,
+               inner_rtol=best["inner_rtol"] if best else 1e-5, trials=trials)
+    cache[key] = out
+    try:
+        json.dump(cache, open(path, "w"), indent=1)
+    except OSError:
+        pass
+    return out

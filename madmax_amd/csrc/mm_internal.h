@@ -84,7 +84,8 @@ struct RveDev {
   int cntA, cntB, status, pad_;
   unsigned done_in, done_out;
   double alpha[MM_MAX_RHS], beta[MM_MAX_RHS], rz[MM_MAX_RHS];
-  double bnorm_in[MM_MAX_RHS], r0[MM_MAX_RHS], rnorm[MM_MAX_RHS];
+  double bnorm_in[MM_MAX_RHS];  // absolute PCG target of the current sweep
+  double r0[MM_MAX_RHS], rnorm[MM_MAX_RHS];
 };
 
 // Results written by the device into host-pinned memory.
@@ -101,6 +102,7 @@ struct SolveParams {
   int nrhs, slab_nodes;
   double rtol, abs_tol, inner_rtol;
   int max_inner_its, max_sweeps;
+  int restart;  // 1: restart PCG every sweep; 0: residual replacement, keep p and q
 };
 
 // Device pointers of the engine arena handed to every solver kernel.
