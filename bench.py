@@ -110,6 +110,12 @@ def main():
     achieved = spmv_bytes / (spmv_ms * 1e-3) / 1e9
     log("spmv: %.3f ms, %.3f GB/launch -> %.0f GB/s" % (spmv_ms, spmv_bytes / 1e9, achieved))
     eng.close()
+    import ctypes
+    copy_ms = ctypes.c_double()
+    copy_bytes = 4 << 30
+    mm.lib().mm_hbm_copy_probe(0, copy_bytes, 10, ctypes.byref(copy_ms))
+    copy_gbs = 2 * copy_bytes / (copy_ms.value * 1e-3) / 1e9
+    log("device copy: %.0f GB/s (read + write of 4 GiB)" % copy_gbs)
 
     # plain FP64 single-RVE GPU path: one load case at a time, host check per iteration
     plain = None
@@ -204,7 +210,8 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
                      "kernel": "k_spmv_time<float,6> (batched ELL SpMV, %d resident RVEs)" % B,
-                     "avg_ms": spmv_ms, "bytes_per_launch": spmv_bytes},
+                     "avg_ms": spmv_ms, "bytes_per_launch": spmv_bytes,
+                     "copy_gbs_measured": copy_gbs},
         "cpu_baseline": cpu,
         "plain_fp64_gpu": plain,
         "speedup_vs_plain": (value / plain["value"]) if plain else None,

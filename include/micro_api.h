@@ -226,6 +226,12 @@ int mm_engine_download_x(mm_engine *eng, uint64_t warm_key, int load, double *x)
  * duration and the algorithmic bytes of one launch. */
 int mm_engine_time_spmv(mm_engine *eng, int reps, double *avg_ms, double *bytes);
 
+/* Copies `bytes` between two device buffers `reps` times (the runtime's
+ * device-to-device copy kernel) and reports the average duration: the
+ * practical HBM ceiling, and the known-size reference that calibrates the
+ * FETCH_SIZE / WRITE_SIZE counters. */
+int mm_hbm_copy_probe(int device, size_t bytes, int reps, double *avg_ms);
+
 #ifdef __cplusplus
 }
 #endif

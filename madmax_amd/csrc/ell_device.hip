@@ -465,20 +465,20 @@ extern "C" int ell_device_solve_pcg(ell_device *d, ell_solver *solver, const dou
 extern "C" int ell_device_destroy(ell_device *d)
 {
   if (!d) return 1;
-  if (d->stream) hipStreamSynchronize(d->stream);
-  hipFree(d->cols);
-  hipFree(d->vals);
-  hipFree(d->dinv);
-  hipFree(d->x);
-  hipFree(d->b);
-  hipFree(d->r);
-  hipFree(d->z);
-  hipFree(d->pA);
-  hipFree(d->pB);
-  hipFree(d->q);
-  hipFree(d->part);
-  hipFree(d->st);
-  if (d->stream) hipStreamDestroy(d->stream);
+  if (d->stream) (void)hipStreamSynchronize(d->stream);
+  (void)hipFree(d->cols);
+  (void)hipFree(d->vals);
+  (void)hipFree(d->dinv);
+  (void)hipFree(d->x);
+  (void)hipFree(d->b);
+  (void)hipFree(d->r);
+  (void)hipFree(d->z);
+  (void)hipFree(d->pA);
+  (void)hipFree(d->pB);
+  (void)hipFree(d->q);
+  (void)hipFree(d->part);
+  (void)hipFree(d->st);
+  if (d->stream) (void)hipStreamDestroy(d->stream);
   free(d);
   return 0;
 }

@@ -119,7 +119,7 @@ template <typename T>
 int ensure_pinned(T **p, size_t *cap, size_t n)
 {
   if (n <= *cap) return 0;
-  if (*p) hipHostFree(*p);
+  if (*p) (void)hipHostFree(*p);
   *p = nullptr;
   size_t c = std::max<size_t>(n, 2 * *cap);
   if (hipHostMalloc((void **)p, c * sizeof(T), hipHostMallocDefault) != hipSuccess) {
@@ -134,7 +134,7 @@ template <typename T>
 int ensure_device(T **p, size_t *cap, size_t n)
 {
   if (n <= *cap) return 0;
-  if (*p) hipFree(*p);
+  if (*p) (void)hipFree(*p);
   *p = nullptr;
   size_t c = std::max<size_t>(n, 2 * *cap);
   if (hipMalloc((void **)p, c * sizeof(T)) != hipSuccess) {
@@ -257,10 +257,10 @@ static int engine_free_arena(mm_engine *e)
                   e->slab_rve, e->slab_node0, e->rves, e->elem_mat, e->ke_tab, e->c_tab, e->bw_tab,
                   e->dv_states, e->dv_slots, e->dv_ke, e->dv_slabs, e->dv_phase};
   for (void *p : ptrs)
-    if (p) hipFree(p);
+    if (p) (void)hipFree(p);
   void *hp[] = {e->out_host, e->st_states, e->st_slots, e->st_ke, e->st_slabs, e->st_phase};
   for (void *p : hp)
-    if (p) hipHostFree(p);
+    if (p) (void)hipHostFree(p);
   return 0;
 }
 
@@ -395,7 +395,7 @@ extern "C" int mm_engine_create(const mm_engine_config *cfg, mm_engine **out)
   return 0;
 fail:
   engine_free_arena(e);
-  if (e->stream) hipStreamDestroy(e->stream);
+  if (e->stream) (void)hipStreamDestroy(e->stream);
   delete e;
   return rc;
 }
@@ -406,11 +406,11 @@ extern "C" int mm_engine_destroy(mm_engine *e)
 {
   if (!e) return MM_ERR_ARG;
   mm_engine_stop(e);
-  if (e->stream) hipStreamSynchronize(e->stream);
-  if (e->gexec) hipGraphExecDestroy(e->gexec);
-  if (e->graph) hipGraphDestroy(e->graph);
+  if (e->stream) (void)hipStreamSynchronize(e->stream);
+  if (e->gexec) (void)hipGraphExecDestroy(e->gexec);
+  if (e->graph) (void)hipGraphDestroy(e->graph);
   engine_free_arena(e);
-  if (e->stream) hipStreamDestroy(e->stream);
+  if (e->stream) (void)hipStreamDestroy(e->stream);
   delete e;
   return 0;
 }
@@ -456,7 +456,7 @@ static void release_slot_resources(mm_engine *e, int slot)
   HostSlot &h = e->slots[slot];
   e->slabs.release(h.first_slab, h.nslab);
   e->elems.release(h.elem_base, h.nelm);
-  hipMemsetAsync(e->slab_rve + h.first_slab, 0xff, (size_t)h.nslab * 4, e->stream);
+  (void)hipMemsetAsync(e->slab_rve + h.first_slab, 0xff, (size_t)h.nslab * 4, e->stream);
   if (h.req.warm_key) {
     auto it = e->warm_map.find(h.req.warm_key);
     if (it != e->warm_map.end() && it->second == slot) e->warm_map.erase(it);
@@ -651,8 +651,8 @@ static int run_steps(mm_engine *e)
   const int T = e->cfg.steps_per_launch;
   if (e->cfg.use_graph) {
     if (!e->gexec || e->graph_grid != grid) {
-      if (e->gexec) hipGraphExecDestroy(e->gexec);
-      if (e->graph) hipGraphDestroy(e->graph);
+      if (e->gexec) (void)hipGraphExecDestroy(e->gexec);
+      if (e->graph) (void)hipGraphDestroy(e->graph);
       e->gexec = nullptr;
       e->graph = nullptr;
       MM_HIP(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
@@ -786,7 +786,7 @@ extern "C" int mm_engine_drain(mm_engine *e)
 
 static void worker_main(mm_engine *e)
 {
-  hipSetDevice(e->cfg.device);
+  (void)hipSetDevice(e->cfg.device);
   while (e->running.load()) {
     if (!has_work(e)) {
       std::unique_lock<std::mutex> lk(e->qmu);
@@ -949,10 +949,51 @@ extern "C" int mm_engine_time_spmv(mm_engine *e, int reps, double *avg_ms, doubl
   MM_HIP(hipEventSynchronize(b));
   float ms = 0.0f;
   MM_HIP(hipEventElapsedTime(&ms, a, b));
-  hipEventDestroy(a);
-  hipEventDestroy(b);
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
   e->spmv_ms = ms / reps;
   if (avg_ms) *avg_ms = e->spmv_ms;
   if (bytes) *bytes = rows * (kW * (double)e->tsz + 27.0 * 4.0 / 3.0 + 2.0 * e->K * e->tsz);
   return 0;
+}
+
+extern "C" int mm_hbm_copy_probe(int device, size_t bytes, int reps, double *avg_ms)
+{
+  if (bytes == 0 || reps < 1 || !avg_ms) return MM_ERR_ARG;
+  MM_HIP(hipSetDevice(device));
+  void *a = nullptr, *b = nullptr;
+  hipStream_t s = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  int rc = 0;
+  float ms = 0.0f;
+  if (hipMalloc(&a, bytes) != hipSuccess || hipMalloc(&b, bytes) != hipSuccess ||
+      hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
+    rc = MM_ERR_HIP;
+    goto done;
+  }
+  if (hipMemsetAsync(a, 1, bytes, s) != hipSuccess ||
+      hipMemcpyAsync(b, a, bytes, hipMemcpyDeviceToDevice, s) != hipSuccess ||
+      hipEventRecord(e0, s) != hipSuccess) {
+    rc = MM_ERR_HIP;
+    goto done;
+  }
+  for (int i = 0; i < reps; i++)
+    if (hipMemcpyAsync(b, a, bytes, hipMemcpyDeviceToDevice, s) != hipSuccess) {
+      rc = MM_ERR_HIP;
+      goto done;
+    }
+  if (hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+      hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
+    rc = MM_ERR_HIP;
+    goto done;
+  }
+  *avg_ms = ms / reps;
+done:
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  if (s) (void)hipStreamDestroy(s);
+  if (a) (void)hipFree(a);
+  if (b) (void)hipFree(b);
+  return rc;
 }
