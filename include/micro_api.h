@@ -147,10 +147,6 @@ typedef struct {
   double inner_rtol;     /* per-sweep PCG target */
   int max_inner_its;
   int max_sweeps;
-  int restart;           /* 1: every refinement sweep restarts PCG from zero
-                            (classic iterative refinement); 0: residual
-                            replacement -- the true FP64 residual replaces r
-                            and PCG continues with the same search direction */
   int steps_per_launch;  /* PCG iterations per launch / graph replay */
   int use_graph;         /* capture the steps in a hipGraph */
   int sync_every_step;   /* plain path: host reads the state after every step */

@@ -72,7 +72,7 @@ class EngineConfig(C.Structure):
     _fields_ = [("device", C.c_int), ("hbm_budget", C.c_size_t), ("max_rves", C.c_int),
                 ("slab_nodes", C.c_int), ("nrhs", C.c_int), ("precision", C.c_int),
                 ("rtol", C.c_double), ("abs_tol", C.c_double), ("inner_rtol", C.c_double),
-                ("max_inner_its", C.c_int), ("max_sweeps", C.c_int), ("restart", C.c_int),
+                ("max_inner_its", C.c_int), ("max_sweeps", C.c_int),
                 ("steps_per_launch", C.c_int), ("use_graph", C.c_int),
                 ("sync_every_step", C.c_int)]
 

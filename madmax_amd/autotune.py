@@ -15,8 +15,8 @@ from . import _lib
 from .engine import Engine, unit_loads
 
 SLAB_NODES = (64, 128, 256)
-INNER_RTOLS = (1e-2, 1e-3, 1e-4)
-BASE_INNER = 1e-3
+INNER_RTOLS = (1e-3, 1e-4, 1e-5, 1e-6)
+BASE_INNER = 1e-5
 
 
 def _cache_path():

@@ -235,7 +235,6 @@ struct mm_engine {
     sp.inner_rtol = cfg.inner_rtol;
     sp.max_inner_its = cfg.max_inner_its;
     sp.max_sweeps = cfg.max_sweeps;
-    sp.restart = cfg.restart;
     return sp;
   }
   double now() const
@@ -276,10 +275,9 @@ extern "C" int mm_engine_config_default(mm_engine_config *c)
   c->precision = MM_PREC_MIXED;
   c->rtol = 1.0e-10;
   c->abs_tol = 0.0;
-  c->inner_rtol = 1.0e-3;
+  c->inner_rtol = 1.0e-5;
   c->max_inner_its = 20000;
   c->max_sweeps = 8;
-  c->restart = 0;
   c->steps_per_launch = 16;
   c->use_graph = 1;
   c->sync_every_step = 0;

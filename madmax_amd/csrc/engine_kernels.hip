@@ -492,52 +492,28 @@ __global__ __launch_bounds__(256) void k_step_b(EngPtrs P, SolveParams sp)
   double s2[2 * K];
   group_reduce<2 * K>(P.part + (size_t)R->first_slab * kPart, nslab, kPart, K, s2, lds);
   if (tid == 0) {
+    // each load case stops at its own sweep target; the sweep ends when all have
     R->its += 1;
     unsigned done = R->done_in;
     const unsigned all = (1u << K) - 1u;
-    if (sp.restart) {
-      // each load case stops at its own sweep target; the sweep ends when all have
-      for (int k = 0; k < K; k++) {
-        const double rr = s2[k], rz = s2[K + k];
-        if ((done >> k) & 1u) {
-          R->beta[k] = 0.0;
-          continue;
-        }
-        if (sqrt(rr) <= R->bnorm_in[k] || R->rz[k] == 0.0) {
-          done |= 1u << k;
-          R->beta[k] = 0.0;
-        } else {
-          R->beta[k] = rz / R->rz[k];
-          R->rz[k] = rz;
-        }
+    for (int k = 0; k < K; k++) {
+      const double rr = s2[k], rz = s2[K + k];
+      if ((done >> k) & 1u) {
+        R->beta[k] = 0.0;
+        continue;
       }
-      R->done_in = done;
-      if ((done & all) == all || R->its >= sp.max_inner_its) {
-        R->its_total += R->its;
-        R->phase = PH_XUPD;
-      }
-    } else {
-      // residual replacement: live load cases keep iterating until all are
-      // below the sweep target; then the true FP64 residual replaces r and
-      // the iteration continues from the same direction (rz is kept at the
-      // value alpha used, for beta after the replacement)
-      bool below = true;
-      for (int k = 0; k < K; k++)
-        if (!((done >> k) & 1u) && !(sqrt(s2[k]) <= R->bnorm_in[k])) below = false;
-      if (below || R->its >= sp.max_inner_its) {
-        R->its_total += R->its;
-        R->phase = PH_XUPD;
+      if (sqrt(rr) <= R->bnorm_in[k] || R->rz[k] == 0.0) {
+        done |= 1u << k;
+        R->beta[k] = 0.0;
       } else {
-        for (int k = 0; k < K; k++) {
-          const double rz = s2[K + k];
-          if (((done >> k) & 1u) || R->rz[k] == 0.0) {
-            R->beta[k] = 0.0;
-          } else {
-            R->beta[k] = rz / R->rz[k];
-            R->rz[k] = rz;
-          }
-        }
+        R->beta[k] = rz / R->rz[k];
+        R->rz[k] = rz;
       }
+    }
+    R->done_in = done;
+    if ((done & all) == all || R->its >= sp.max_inner_its) {
+      R->its_total += R->its;
+      R->phase = PH_XUPD;
     }
   }
 }
@@ -546,7 +522,8 @@ __global__ __launch_bounds__(256) void k_step_b(EngPtrs P, SolveParams sp)
 // Residual kernel: PH_RES, the FP64 residual r = -K x64 computed matrix-free
 // from the element stiffness tables (the Newton / refinement residual of
 // micro/src/assembly.c:33-82; boundary rows keep a zero residual, :59-74),
-// then the start of the next PCG sweep (d = 0, z = dinv r, p = q = 0).
+// then the start of the next PCG sweep (d = 0, z = dinv r, p = q = 0): every
+// refinement sweep solves the correction equation from zero.
 // The relative target refers to the cold-start right-hand side (interior
 // displacement zero, the ||b|| of the first solve); a warm-started RVE
 // measures it at sweep 0 from the boundary neighbours alone.
@@ -566,9 +543,6 @@ __global__ __launch_bounds__(256) void k_step_res(EngPtrs P, SolveParams sp)
   const int nslab = R->nslab;
   double *part = P.part + (size_t)slab * kPart;
   const bool need_cold = R->warm && R->sweep == 0;
-  // a fresh PCG start (first sweep, or every sweep when restarting) clears
-  // p and q; a residual replacement keeps them
-  const bool fresh = sp.restart || R->sweep == 0;
   double v[4 * K];
 #pragma unroll
   for (int i = 0; i < 4 * K; i++) v[i] = 0.0;
@@ -637,10 +611,8 @@ __global__ __launch_bounds__(256) void k_step_res(EngPtrs P, SolveParams sp)
       store_vec<TS, KS>(rv + grow * KS, ri);
       store_vec<TS, KS>(zv + grow * KS, zi);
       store_vec<TS, KS>(dv + grow * KS, zero);
-      if (fresh) {
-        store_vec<TS, KS>((TS *)P.p + grow * KS, zero);
-        store_vec<TS, KS>((TS *)P.q + grow * KS, zero);
-      }
+      store_vec<TS, KS>((TS *)P.p + grow * KS, zero);
+      store_vec<TS, KS>((TS *)P.q + grow * KS, zero);
     }
   }
   block_sum<4 * K>(v, lds);
@@ -663,7 +635,7 @@ __global__ __launch_bounds__(256) void k_step_res(EngPtrs P, SolveParams sp)
       // side, but no tighter than what the outer tolerance still needs
       const double cap = (R->r0[k] > 0.0 && rn > 0.0) ? 0.5 * sp.rtol * R->r0[k] / rn : 0.0;
       R->bnorm_in[k] = fmax(sp.inner_rtol, cap) * sqrt(s[1]);
-      R->beta[k] = (fresh || conv || R->rz[k] == 0.0) ? 0.0 : s[2] / R->rz[k];
+      R->beta[k] = 0.0;
       R->rz[k] = s[2];
       R->alpha[k] = 0.0;
     }

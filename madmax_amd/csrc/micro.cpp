@@ -110,12 +110,11 @@ extern "C" int micro_init(const micro_config *cfg)
     ec.inner_rtol = cfg->rtol;
     ec.max_inner_its = 100000;
     ec.max_sweeps = cfg->nl_max_its;
-    ec.restart = 1;  // every Newton iteration solves afresh, like homog_fe2
   } else {
     ec.precision = MM_PREC_MIXED;
     ec.rtol = cfg->rtol;
     ec.abs_tol = cfg->nl_min_norm;
-    ec.inner_rtol = 1.0e-3;
+    ec.inner_rtol = 1.0e-5;
     ec.max_inner_its = 100000;
     ec.max_sweeps = cfg->nl_max_its > 12 ? cfg->nl_max_its : 12;
   }

@@ -102,7 +102,6 @@ struct SolveParams {
   int nrhs, slab_nodes;
   double rtol, abs_tol, inner_rtol;
   int max_inner_its, max_sweeps;
-  int restart;  // 1: restart PCG every sweep; 0: residual replacement, keep p and q
 };
 
 // Device pointers of the engine arena handed to every solver kernel.

@@ -83,9 +83,9 @@ class Engine:
     """Batched / serving RVE engine (mm_engine_*)."""
 
     def __init__(self, nrhs=6, precision=_lib.MM_PREC_MIXED, hbm_budget=0, max_rves=0,
-                 slab_nodes=0, rtol=1e-10, abs_tol=0.0, inner_rtol=1e-3, max_inner_its=20000,
+                 slab_nodes=0, rtol=1e-10, abs_tol=0.0, inner_rtol=1e-5, max_inner_its=20000,
                  max_sweeps=8, steps_per_launch=16, use_graph=True, sync_every_step=False,
-                 restart=False, device=0):
+                 device=0):
         L = lib()
         c = EngineConfig()
         check(L.mm_engine_config_default(C.byref(c)), "mm_engine_config_default")
@@ -100,7 +100,6 @@ class Engine:
         c.inner_rtol = inner_rtol
         c.max_inner_its = int(max_inner_its)
         c.max_sweeps = int(max_sweeps)
-        c.restart = int(bool(restart))
         c.steps_per_launch = int(steps_per_launch)
         c.use_graph = int(bool(use_graph))
         c.sync_every_step = int(bool(sync_every_step))

@@ -53,7 +53,7 @@ def test_batched_uneven_tangents_vs_oracle(precision):
     specs = [configs.c4_spec(4, i) for i in range(6)]
     specs = [s for s in specs if s.nex <= 20][:4] + [rve_spec(7, [(0.45, 0.5, 0.33)])]
     with Engine(nrhs=6, precision=precision, hbm_budget=2 * GB, max_rves=64,
-                inner_rtol=1e-10 if precision == MM_PREC_FP64 else 1e-3) as eng:
+                inner_rtol=1e-10 if precision == MM_PREC_FP64 else 1e-5) as eng:
         out = _solve_batch(eng, specs)
     for s, r in zip(specs, out):
         assert r["status"] == 0
@@ -166,19 +166,20 @@ def test_c2_tangent_30cube_vs_oracle():
     assert normwise_rel(r["c_tangent"], c_ref) <= TOL
 
 
-def test_residual_replacement_vs_restarted_refinement():
-    """Both mixed-precision schemes reach the FP64 target; replacing the
-    residual and keeping the search direction needs fewer PCG iterations than
-    restarting PCG every refinement sweep."""
+def test_refinement_sweep_targets():
+    """Looser FP32 sweep targets need more FP64 refinement sweeps but reach the
+    same FP64 tolerance; the last sweep's target is capped by what the outer
+    tolerance still needs, so a tight target does not over-solve."""
     specs = [configs.c2_spec(2, i, n=16) for i in range(3)]
     out = {}
-    for restart in (True, False):
-        with Engine(nrhs=6, hbm_budget=GB, max_rves=8, inner_rtol=1e-4, rtol=1e-10,
-                    restart=restart, max_sweeps=12) as eng:
-            out[restart] = _solve_batch(eng, specs)
-    for s, a, b in zip(specs, out[True], out[False]):
+    for tol in (1e-2, 1e-5):
+        with Engine(nrhs=6, hbm_budget=GB, max_rves=8, inner_rtol=tol, rtol=1e-10,
+                    max_sweeps=16) as eng:
+            out[tol] = _solve_batch(eng, specs)
+    for s, a, b in zip(specs, out[1e-2], out[1e-5]):
         c_ref = _oracle_tangent(s)
         assert a["status"] == 0 and b["status"] == 0
         assert normwise_rel(a["c_tangent"], c_ref) <= TOL
         assert normwise_rel(b["c_tangent"], c_ref) <= TOL
-        assert b["pcg_its"] <= a["pcg_its"]
+        assert a["sweeps"] > b["sweeps"]
+        assert a["residual_rel"] <= 1e-10 and b["residual_rel"] <= 1e-10
