@@ -333,6 +333,11 @@ extern "C" int mm_engine_create(const mm_engine_config *cfg, mm_engine **out)
     return MM_ERR_ALLOC;
   }
   e->nslab_cap = (long long)((budget - (size_t)e->rve_cap * per_rve) / e->bytes_per_slab());
+  e->nslab_cap -= e->nslab_cap % 8;  // solver grids are multiples of 8 (XCD-aware order)
+  if (e->nslab_cap < 8) {
+    delete e;
+    return MM_ERR_ALLOC;
+  }
   e->elem_cap = e->nslab_cap * e->SN + 64;
   const size_t nrows = (size_t)e->nslab_cap * e->rows_per_slab();
   const size_t slots = nrows * kW;
@@ -508,7 +513,7 @@ static void fill_state(const mm_engine *e, const HostSlot &h, RveDev *st, int wa
   st->warm = warm;
   for (int k = 0; k < h.req.nload; k++)
     for (int i = 0; i < MM_NVOI; i++) st->load[k][i] = h.req.strain[k][i];
-  st->phase = PH_RES;
+  for (int c = 0; c < 3; c++) st->st[c].phase = PH_RES;
 }
 
 // Admits queued requests that fit; uploads their state and assembles them.
@@ -638,12 +643,21 @@ static int admit(mm_engine *e)
   return 0;
 }
 
+// Slabs launched: the allocated ones rounded up to a multiple of 64 (whole
+// waves of workgroups per XCD), never beyond the arena, and always a multiple
+// of 8 so the XCD-aware slab order is a permutation of the grid.
+static int solver_grid(const mm_engine *e, long long hw)
+{
+  long long g = ((hw + 63) / 64) * 64;
+  if (g > e->nslab_cap) g = e->nslab_cap;
+  return (int)g;
+}
+
 static int run_steps(mm_engine *e)
 {
   long long hw = e->slabs.high_water();
   if (hw <= 0) return 0;
-  int grid = (int)(((hw + 63) / 64) * 64);
-  if (grid > e->nslab_cap) grid = (int)e->nslab_cap;
+  const int grid = solver_grid(e, hw);
   EngPtrs P = e->ptrs();
   SolveParams sp = e->params();
   const int T = e->cfg.steps_per_launch;
@@ -925,7 +939,7 @@ extern "C" int mm_engine_time_spmv(mm_engine *e, int reps, double *avg_ms, doubl
   std::lock_guard<std::mutex> lk(e->step_mu);
   long long hw = e->slabs.high_water();
   if (hw <= 0 || e->n_active > 0) return MM_ERR_STATE;  // it overwrites q
-  const int grid = (int)hw;
+  const int grid = solver_grid(e, hw);
   EngPtrs P = e->ptrs();
   // algorithmic bytes of one launch: every resident row reads its 81 values,
   // every node its 27 block columns, and the vector z is read and w written

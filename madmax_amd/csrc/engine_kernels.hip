@@ -18,10 +18,11 @@
 // first_slab * 3 * SN.
 //
 // The solver kernels advance every resident RVE by one phase step; their grid
-// covers all slabs and each workgroup reads the phase of its RVE.  The
-// workgroup that finishes last for an RVE (arrive_last) reduces that RVE's
-// partials in slab order, so an RVE's results do not depend on which other
-// RVEs share the launch.
+// covers all slabs and each workgroup reads the phase of its RVE.  Dot
+// products are finished by every workgroup of the next launch from the
+// per-slab partials, summed in slab order, so an RVE's results do not depend
+// on which other RVEs share the launch, and no workgroup fences for another
+// (an agent-scope release in every workgroup costs an L2 write-back each).
 #include <hip/hip_runtime.h>
 
 #include "mm_device.h"
@@ -43,6 +44,17 @@ int upload_slot_tables(hipStream_t s)
 constexpr int kPart = kPartDoubles;
 
 __host__ __device__ constexpr int vec_stride(int K) { return (K == 1) ? 1 : ((K + 1) & ~1); }
+
+// MI355X dispatches workgroup i to XCD i % 8.  The solver grids are a
+// multiple of 8 and workgroup i processes slab (i % 8) * (grid / 8) + i / 8,
+// so each XCD walks one contiguous run of slabs and the z rows that
+// neighbouring slabs of an RVE share stay in that XCD's L2.
+constexpr int kXcd = 8;
+__device__ __forceinline__ int xcd_slab()
+{
+  const int per = gridDim.x / kXcd;
+  return (blockIdx.x % kXcd) * per + blockIdx.x / kXcd;
+}
 
 __device__ __forceinline__ int node_class(int ix, int iy, int iz, int nx, int ny, int nz)
 {
@@ -355,194 +367,378 @@ __device__ __forceinline__ void node_spmv(const int *__restrict__ bcols, const T
 }
 
 // ---------------------------------------------------------------------------
-// Kernel A: PH_INNER and PH_XUPD.
-// INNER: w = A z, then p = z + beta p and q = w + beta q (so q = A p without
-// gathering p), and p.q; the last workgroup forms alpha = r.z / p.q.
-// XUPD: x64 += d.
+// Element-wise passes over a slab's vectors.  A slab's rows are contiguous in
+// every [row][KS] vector, so lane g of the workgroup handles VEC consecutive
+// values g*VEC.. of the slab (VEC = 2 for even KS): each vector access of a
+// wavefront is one contiguous 512-byte (FP32) run.  The load case of value
+// i is i % KS; per-load-case scalars come from LDS.
+template <int KS>
+struct SlabVec {
+  static constexpr int VEC = (KS % 2 == 0) ? 2 : 1;
+};
+
+// Adds a[j] to acc[k0 + j] for the (VEC) load cases k0.. without dynamic
+// register indexing.
+template <int K, int KS, int VEC>
+__device__ __forceinline__ void acc_by_case(double (&acc)[K], int k0, const double (&a)[VEC])
+{
+#pragma unroll
+  for (int kk = 0; kk < KS; kk += VEC)
+    if (kk == k0)
+#pragma unroll
+      for (int j = 0; j < VEC; j++)
+        if (kk + j < K) acc[(kk + j < K) ? kk + j : 0] += a[j];
+}
+
+// Per-slab partial sums of the launch that produced them, summed over the
+// RVE's slabs in slab order (group_sum): every workgroup of the consuming
+// launch forms the same totals.  Regions of a slab's kPart doubles:
+//   [0, K)      p.q            (A -> B)
+//   [K, 3K)     r.r, r.z       (B -> A)
+//   [3K, 7K)    residual norms (R -> A)
+//   [0, 12K)    averages       (R -> A), never pending with the others
+__device__ __forceinline__ double *slab_part(const EngPtrs &P, int slab)
+{
+  return P.part + (size_t)slab * kPart;
+}
+
+// Start of a refinement sweep from the reduced residual norms s4 =
+// {||r64||^2, ||r||^2, r.z, ||r_cold||^2} per load case (FP64 residual r64,
+// its rounding r, cold-start right-hand side r_cold).
+template <int K>
+__device__ void start_sweep(SolveState &N, const double (&s4)[4 * K], const RveDev *R,
+                            const SolveParams &sp)
+{
+  const bool need_cold = R->warm && N.sweep == 0;
+  unsigned done_out = 0;
+  for (int k = 0; k < K; k++) {
+    const double rn = sqrt(s4[k]);
+    N.rnorm[k] = rn;
+    if (N.sweep == 0) N.r0[k] = need_cold ? sqrt(s4[3 * K + k]) : rn;
+    const bool conv = (k >= R->nload) || N.r0[k] == 0.0 || rn <= sp.rtol * N.r0[k] ||
+                      rn < sp.abs_tol || N.sweep >= sp.max_sweeps;
+    if (conv) done_out |= 1u << k;
+    // absolute target of this sweep: inner_rtol of the sweep's right-hand
+    // side, but no tighter than what the outer tolerance still needs
+    const double cap = (N.r0[k] > 0.0 && rn > 0.0) ? 0.5 * sp.rtol * N.r0[k] / rn : 0.0;
+    N.bnorm_in[k] = fmax(sp.inner_rtol, cap) * sqrt(s4[K + k]);
+    N.beta[k] = 0.0;
+    N.rz[k] = s4[2 * K + k];
+  }
+  const unsigned all = (1u << K) - 1u;
+  N.done_out = done_out;
+  if ((done_out & all) == all) {
+    N.phase = PH_AVG;
+  } else {
+    N.done_in = done_out;
+    N.its = 0;
+    N.phase = PH_INNER;
+  }
+}
+
+// End of a PCG iteration from the reduced s2 = {r.r, r.z}: each load case
+// stops at its own sweep target; the sweep ends (x += d, then a new FP64
+// residual) when all have, or at max_inner_its.  Returns true at sweep end.
+template <int K>
+__device__ bool inner_check(SolveState &N, const double (&s2)[2 * K], const SolveParams &sp)
+{
+  N.its += 1;
+  unsigned done = N.done_in;
+  const unsigned all = (1u << K) - 1u;
+  for (int k = 0; k < K; k++) {
+    const double rr = s2[k], rz = s2[K + k];
+    if ((done >> k) & 1u) {
+      N.beta[k] = 0.0;
+      continue;
+    }
+    if (sqrt(rr) <= N.bnorm_in[k] || N.rz[k] == 0.0) {
+      done |= 1u << k;
+      N.beta[k] = 0.0;
+    } else {
+      N.beta[k] = rz / N.rz[k];
+      N.rz[k] = rz;
+    }
+  }
+  N.done_in = done;
+  if ((done & all) == all || N.its >= sp.max_inner_its) {
+    N.its_total += N.its;
+    N.sweep += 1;
+    N.phase = PH_RES;
+    return true;
+  }
+  return false;
+}
+
+// Volume averages and tangent of a finished RVE (micro/src/homogenize.c:
+// 344-368, :80) from the averaging partials; run by one workgroup.
+template <int K>
+__device__ void finish_rve(const EngPtrs &P, int slot, const RveDev *R, SolveState &N,
+                           const SolveParams &sp, double *lds)
+{
+  __shared__ double tot[MM_MAX_RHS][12];
+  const double *p0 = slab_part(P, R->first_slab);
+#pragma unroll 1
+  for (int k = 0; k < K; k++) {
+    double s12[12];
+    group_sum<12>(p0, R->nslab, kPart, k * 12, s12, lds);
+    if (threadIdx.x == 0)
+      for (int i = 0; i < 12; i++) tot[k][i] = s12[i];
+  }
+  if (threadIdx.x != 0) return;
+  RveOut *o = P.out + slot;
+  double sig[MM_MAX_RHS][6];
+  double rel = 0.0;
+  bool ok = true;
+#pragma unroll 1
+  for (int k = 0; k < K; k++) {
+#pragma unroll 1
+    for (int i = 0; i < 12; i++) {
+      const double s = tot[k][i] / R->vol;
+      if (i < 6) o->strain[k][i] = s;
+      else {
+        o->stress[k][i - 6] = s;
+        sig[k][i - 6] = s;
+      }
+    }
+    if (k < R->nload) {
+      const double r0 = N.r0[k], rn = N.rnorm[k];
+      rel = fmax(rel, (r0 > 0.0) ? rn / r0 : 0.0);
+      if (!(r0 == 0.0 || rn <= sp.rtol * r0 || rn < sp.abs_tol)) ok = false;
+    }
+  }
+  for (int i = 0; i < 36; i++) o->c[i] = 0.0;
+  if (R->tangent_mode == MM_TANGENT_UNIT && K >= 6) {
+    for (int i = 0; i < 6; i++)
+      for (int j = 0; j < 6; j++) o->c[j * 6 + i] = sig[i][j] / R->load[i][i];
+  } else if (R->tangent_mode == MM_TANGENT_FD && K >= 7) {
+    for (int i = 0; i < 6; i++)
+      for (int j = 0; j < 6; j++)
+        o->c[j * 6 + i] = (sig[i + 1][j] - sig[0][j]) / (R->load[i + 1][i] - R->load[0][i]);
+  }
+  o->residual_rel = rel;
+  o->its_total = N.its_total;
+  o->sweeps = N.sweep;
+  o->status = ok ? MM_OK : MM_NOT_CONVERGED;
+  N.status = o->status;
+  N.phase = PH_DONE;
+  __threadfence_system();
+  o->phase = PH_DONE;
+  __threadfence_system();
+}
+
+// ---------------------------------------------------------------------------
+// Kernel A (reads st[2], writes st[0]).
+// PH_RESRED: starts the sweep from the residual norms, then continues as
+// INNER with beta = 0.  PH_INNER: from the last B's r.r and r.z, either ends
+// the sweep (x64 += d, PH_RES) or forms beta and does w = A z (one lane per
+// node, staged in LDS), p = z + beta p, q = w + beta q (so q = A p without
+// gathering p) and the partial p.q.  PH_AVGRED: the first slab's workgroup
+// writes the result.
 template <typename TS, int K>
 __global__ __launch_bounds__(256) void k_step_a(EngPtrs P, SolveParams sp)
 {
   constexpr int KS = vec_stride(K);
-  __shared__ double lds[K * 16];
-  const int slab = blockIdx.x, tid = threadIdx.x, SN = P.SN;
+  constexpr int VEC = SlabVec<KS>::VEC;
+  extern __shared__ __align__(16) unsigned char smem_a[];
+  TS *ws = reinterpret_cast<TS *>(smem_a);  // [3*SN][KS]
+  __shared__ double lds[4 * K * 16];
+  __shared__ SolveState N;
+  __shared__ TS s_beta[KS];
+  __shared__ int s_xupd;
+  const int slab = xcd_slab(), tid = threadIdx.x, SN = P.SN;
   const int slot = P.slab_rve[slab];
   if (slot < 0) return;
   RveDev *R = P.rves + slot;
-  const int phase = R->phase;
-  if (phase != PH_INNER && phase != PH_XUPD) return;
-  const int n = P.slab_node0[slab] + tid;
-  const bool valid = n < R->nrow / 3;
-  const long long rbase = R->row_base;
-  const long long grow0 = rbase + 3LL * n;
+  const bool writer = slab == R->first_slab;
   const int nslab = R->nslab;
-  double *part = P.part + (size_t)slab * kPart;
+  const int phase0 = R->st[2].phase;
+  const double *p0 = slab_part(P, R->first_slab);
+  if (phase0 == PH_AVGRED) {
+    if (!writer) return;
+    if (tid == 0) N = R->st[2];
+    __syncthreads();
+    finish_rve<K>(P, slot, R, N, sp, lds);
+    if (tid == 0) R->st[0] = N;
+    return;
+  }
+  if (tid == 0) {
+    N = R->st[2];
+    s_xupd = 0;
+  }
+  if (phase0 == PH_RESRED) {
+    double s4[4 * K];
+    group_sum<4 * K>(p0, nslab, kPart, 3 * K, s4, lds);
+    if (tid == 0) start_sweep<K>(N, s4, R, sp);
+  } else if (phase0 == PH_INNER) {
+    double s2[2 * K];
+    group_sum<2 * K>(p0, nslab, kPart, K, s2, lds);
+    if (tid == 0) s_xupd = inner_check<K>(N, s2, sp) ? 1 : 0;
+  }
+  __syncthreads();
+  if (writer && tid == 0) R->st[0] = N;
+  const int phase = N.phase;
+  if (phase != PH_INNER && !s_xupd) return;
 
-  if (phase == PH_INNER) {
-    TS *p = (TS *)P.p;
-    TS *q = (TS *)P.q;
-    TS beta[KS];
+  const int n0 = P.slab_node0[slab];
+  const int rows = min(3 * SN, R->nrow - 3 * n0);  // valid rows of this slab
+  const long long rbase = R->row_base;
+  const long long f0 = (rbase + 3LL * n0) * KS;   // first value of the slab
+  const int ngroups = rows * KS / VEC;
+
+  if (s_xupd) {
+    const TS *d = (const TS *)P.d;
+    for (int g = tid; g < ngroups; g += blockDim.x)
 #pragma unroll
-    for (int k = 0; k < KS; k++) beta[k] = (k < K) ? (TS)R->beta[k] : (TS)0;
-    double v[K];
-#pragma unroll
-    for (int k = 0; k < K; k++) v[k] = 0.0;
-    if (valid) {
-      TS w[3][KS];
-      node_spmv<TS, KS>(P.bcols, (const TS *)P.vals, (long long)slab * 27 * SN, (long long)slab * kW * 3 * SN,
-                        SN, tid, rbase, (const TS *)P.z, w);
-      TS zi[3 * KS], pi[3 * KS], qi[3 * KS];
-      load_vec<TS, 3 * KS>((const TS *)P.z + grow0 * KS, zi);
-      load_vec<TS, 3 * KS>(p + grow0 * KS, pi);
-      load_vec<TS, 3 * KS>(q + grow0 * KS, qi);
-#pragma unroll
-      for (int d = 0; d < 3; d++)
-#pragma unroll
-        for (int k = 0; k < KS; k++) {
-          const int i = d * KS + k;
-          pi[i] = fma(beta[k], pi[i], zi[i]);
-          qi[i] = fma(beta[k], qi[i], w[d][k]);
-          if (k < K) v[k < K ? k : 0] += (double)pi[i] * (double)qi[i];
-        }
-      store_vec<TS, 3 * KS>(p + grow0 * KS, pi);
-      store_vec<TS, 3 * KS>(q + grow0 * KS, qi);
-    }
-    block_sum<K>(v, lds);
-    if (tid == 0)
-      for (int k = 0; k < K; k++) part[k] = v[k];
-    if (arrive_last(&R->cntA, nslab)) {
-      double s[K];
-      group_reduce<K>(P.part + (size_t)R->first_slab * kPart, nslab, kPart, 0, s, lds);
-      if (tid == 0)
-        for (int k = 0; k < K; k++) {
-          const bool done = (R->done_in >> k) & 1u;
-          R->alpha[k] = (done || s[k] == 0.0) ? 0.0 : R->rz[k] / s[k];
-        }
-    }
+      for (int j = 0; j < VEC; j++) P.x64[f0 + (long long)g * VEC + j] += (double)d[f0 + (long long)g * VEC + j];
     return;
   }
 
-  // PH_XUPD
-  if (valid) {
-    const TS *d = (const TS *)P.d;
-    for (int r = 0; r < 3; r++)
+  // PH_INNER
+  if (tid < KS) s_beta[tid] = (tid < K) ? (TS)N.beta[tid] : (TS)0;
+  const int n = n0 + tid;
+  if (n < R->nrow / 3) {
+    TS w[3][KS];
+    node_spmv<TS, KS>(P.bcols, (const TS *)P.vals, (long long)slab * 27 * SN,
+                      (long long)slab * kW * 3 * SN, SN, tid, rbase, (const TS *)P.z, w);
 #pragma unroll
-      for (int k = 0; k < K; k++) P.x64[(grow0 + r) * KS + k] += (double)d[(grow0 + r) * KS + k];
+    for (int d = 0; d < 3; d++)
+#pragma unroll
+      for (int k = 0; k < KS; k++) ws[(3 * tid + d) * KS + k] = w[d][k];
   }
-  if (arrive_last(&R->cntA, nslab) && tid == 0) {
-    R->sweep += 1;
-    R->phase = PH_RES;
+  __syncthreads();
+  TS *p = (TS *)P.p;
+  TS *q = (TS *)P.q;
+  const TS *z = (const TS *)P.z;
+  double v[K];
+#pragma unroll
+  for (int k = 0; k < K; k++) v[k] = 0.0;
+  for (int g = tid; g < ngroups; g += blockDim.x) {
+    const long long i0 = f0 + (long long)g * VEC;
+    const int k0 = (g * VEC) % KS;
+    TS zi[VEC], pi[VEC], qi[VEC];
+    load_vec<TS, VEC>(z + i0, zi);
+    load_vec<TS, VEC>(p + i0, pi);
+    load_vec<TS, VEC>(q + i0, qi);
+    double a[VEC];
+#pragma unroll
+    for (int j = 0; j < VEC; j++) {
+      const TS b = s_beta[k0 + j];
+      pi[j] = fma(b, pi[j], zi[j]);
+      qi[j] = fma(b, qi[j], ws[g * VEC + j]);
+      a[j] = (double)pi[j] * (double)qi[j];
+    }
+    store_vec<TS, VEC>(p + i0, pi);
+    store_vec<TS, VEC>(q + i0, qi);
+    acc_by_case<K, KS, VEC>(v, k0, a);
+  }
+  block_sum<K>(v, lds);
+  if (tid == 0) {
+    double *part = slab_part(P, slab);
+    for (int k = 0; k < K; k++) part[k] = v[k];
   }
 }
 
 // ---------------------------------------------------------------------------
-// Kernel B: PH_INNER (d += alpha p, r -= alpha q, z = dinv r, r.r, r.z); the
-// last workgroup checks the per-load-case convergence and forms beta.
+// Kernel B (reads st[0], writes st[1]).  PH_INNER: alpha = r.z / p.q from
+// the last A's partials, then over the slab's values d += alpha p,
+// r -= alpha q, z = dinv r and the partial r.r, r.z.
 template <typename TS, int K>
 __global__ __launch_bounds__(256) void k_step_b(EngPtrs P, SolveParams sp)
 {
   constexpr int KS = vec_stride(K);
+  constexpr int VEC = SlabVec<KS>::VEC;
   __shared__ double lds[2 * K * 16];
-  const int slab = blockIdx.x, tid = threadIdx.x;
+  __shared__ TS s_alpha[KS];
+  const int slab = xcd_slab(), tid = threadIdx.x;
   const int slot = P.slab_rve[slab];
   if (slot < 0) return;
   RveDev *R = P.rves + slot;
-  if (R->phase != PH_INNER) return;
-  const int n = P.slab_node0[slab] + tid;
-  const bool valid = n < R->nrow / 3;
-  const long long grow0 = R->row_base + 3LL * n;
+  const SolveState *S = &R->st[0];
+  if (slab == R->first_slab && tid == 0) R->st[1] = *S;
+  if (S->phase != PH_INNER) return;
   const int nslab = R->nslab;
-  double *part = P.part + (size_t)slab * kPart;
-  double v[2 * K];
-#pragma unroll
-  for (int i = 0; i < 2 * K; i++) v[i] = 0.0;
-  if (valid) {
-    TS *dv = (TS *)P.d, *rv = (TS *)P.r, *zv = (TS *)P.z;
-    const TS *p = (const TS *)P.p, *q = (const TS *)P.q, *dinv = (const TS *)P.dinv;
-    TS alpha[KS];
-#pragma unroll
-    for (int k = 0; k < KS; k++) alpha[k] = (k < K) ? (TS)R->alpha[k] : (TS)0;
-    TS di[3 * KS], ri[3 * KS], pi[3 * KS], qi[3 * KS], zi[3 * KS];
-    load_vec<TS, 3 * KS>(dv + grow0 * KS, di);
-    load_vec<TS, 3 * KS>(rv + grow0 * KS, ri);
-    load_vec<TS, 3 * KS>(p + grow0 * KS, pi);
-    load_vec<TS, 3 * KS>(q + grow0 * KS, qi);
-#pragma unroll
-    for (int d = 0; d < 3; d++) {
-      const TS dg = dinv[grow0 + d];
-#pragma unroll
-      for (int k = 0; k < KS; k++) {
-        const int i = d * KS + k;
-        di[i] = fma(alpha[k], pi[i], di[i]);
-        ri[i] = fma(-alpha[k], qi[i], ri[i]);
-        zi[i] = dg * ri[i];
-        if (k < K) {
-          v[k < K ? k : 0] += (double)ri[i] * (double)ri[i];
-          v[K + (k < K ? k : 0)] += (double)ri[i] * (double)zi[i];
-        }
-      }
-    }
-    store_vec<TS, 3 * KS>(dv + grow0 * KS, di);
-    store_vec<TS, 3 * KS>(rv + grow0 * KS, ri);
-    store_vec<TS, 3 * KS>(zv + grow0 * KS, zi);
-  }
-  block_sum<2 * K>(v, lds);
+  double s1[K];
+  group_sum<K>(slab_part(P, R->first_slab), nslab, kPart, 0, s1, lds);
+  if (tid < KS) s_alpha[tid] = (TS)0;
+  __syncthreads();
   if (tid == 0)
-    for (int i = 0; i < 2 * K; i++) part[K + i] = v[i];
-  if (!arrive_last(&R->cntB, nslab)) return;
-  double s2[2 * K];
-  group_reduce<2 * K>(P.part + (size_t)R->first_slab * kPart, nslab, kPart, K, s2, lds);
-  if (tid == 0) {
-    // each load case stops at its own sweep target; the sweep ends when all have
-    R->its += 1;
-    unsigned done = R->done_in;
-    const unsigned all = (1u << K) - 1u;
     for (int k = 0; k < K; k++) {
-      const double rr = s2[k], rz = s2[K + k];
-      if ((done >> k) & 1u) {
-        R->beta[k] = 0.0;
-        continue;
-      }
-      if (sqrt(rr) <= R->bnorm_in[k] || R->rz[k] == 0.0) {
-        done |= 1u << k;
-        R->beta[k] = 0.0;
-      } else {
-        R->beta[k] = rz / R->rz[k];
-        R->rz[k] = rz;
-      }
+      const bool done = (S->done_in >> k) & 1u;
+      s_alpha[k] = (TS)((done || s1[k] == 0.0) ? 0.0 : S->rz[k] / s1[k]);
     }
-    R->done_in = done;
-    if ((done & all) == all || R->its >= sp.max_inner_its) {
-      R->its_total += R->its;
-      R->phase = PH_XUPD;
+  __syncthreads();
+  const int n0 = P.slab_node0[slab];
+  const int rows = min(3 * P.SN, R->nrow - 3 * n0);
+  const long long row0 = R->row_base + 3LL * n0;
+  const long long f0 = row0 * KS;
+  const int ngroups = rows * KS / VEC;
+  TS *dv = (TS *)P.d, *rv = (TS *)P.r, *zv = (TS *)P.z;
+  const TS *p = (const TS *)P.p, *q = (const TS *)P.q, *dinv = (const TS *)P.dinv;
+  double v[K], u[K];
+#pragma unroll
+  for (int k = 0; k < K; k++) {
+    v[k] = 0.0;
+    u[k] = 0.0;
+  }
+  for (int g = tid; g < ngroups; g += blockDim.x) {
+    const long long i0 = f0 + (long long)g * VEC;
+    const int k0 = (g * VEC) % KS;
+    const TS dg = dinv[row0 + (g * VEC) / KS];
+    TS di[VEC], ri[VEC], pi[VEC], qi[VEC], zi[VEC];
+    load_vec<TS, VEC>(dv + i0, di);
+    load_vec<TS, VEC>(rv + i0, ri);
+    load_vec<TS, VEC>(p + i0, pi);
+    load_vec<TS, VEC>(q + i0, qi);
+    double arr[VEC], arz[VEC];
+#pragma unroll
+    for (int j = 0; j < VEC; j++) {
+      const TS a = s_alpha[k0 + j];
+      di[j] = fma(a, pi[j], di[j]);
+      ri[j] = fma(-a, qi[j], ri[j]);
+      zi[j] = dg * ri[j];
+      arr[j] = (double)ri[j] * (double)ri[j];
+      arz[j] = (double)ri[j] * (double)zi[j];
     }
+    store_vec<TS, VEC>(dv + i0, di);
+    store_vec<TS, VEC>(rv + i0, ri);
+    store_vec<TS, VEC>(zv + i0, zi);
+    acc_by_case<K, KS, VEC>(v, k0, arr);
+    acc_by_case<K, KS, VEC>(u, k0, arz);
+  }
+  double vu[2 * K];
+#pragma unroll
+  for (int k = 0; k < K; k++) {
+    vu[k] = v[k];
+    vu[K + k] = u[k];
+  }
+  block_sum<2 * K>(vu, lds);
+  if (tid == 0) {
+    double *part = slab_part(P, slab);
+    for (int i = 0; i < 2 * K; i++) part[K + i] = vu[i];
   }
 }
 
 // ---------------------------------------------------------------------------
-// Residual kernel: PH_RES, the FP64 residual r = -K x64 computed matrix-free
-// from the element stiffness tables (the Newton / refinement residual of
+// Kernel R (reads st[1], writes st[2]).
+// PH_RES: the FP64 residual r = -K x64, computed matrix-free from the element
+// stiffness tables (the Newton / refinement residual of
 // micro/src/assembly.c:33-82; boundary rows keep a zero residual, :59-74),
-// then the start of the next PCG sweep (d = 0, z = dinv r, p = q = 0): every
-// refinement sweep solves the correction equation from zero.
-// The relative target refers to the cold-start right-hand side (interior
-// displacement zero, the ||b|| of the first solve); a warm-started RVE
-// measures it at sweep 0 from the boundary neighbours alone.
+// and the start of the next PCG sweep (d = 0, z = dinv r, p = q = 0): every
+// refinement sweep solves the correction equation from zero.  The relative
+// target refers to the cold-start right-hand side (interior displacement
+// zero, the ||b|| of the first solve); a warm-started RVE measures it at
+// sweep 0 from the boundary neighbours alone.
+// PH_AVG: per-slab partial volume averages of strain and stress in FP64,
+// sum_gp (.) w_gp per element through sum_gp B w, one load case at a time.
 template <typename TS, int K>
-__global__ __launch_bounds__(256) void k_step_res(EngPtrs P, SolveParams sp)
+__device__ void res_slab(const EngPtrs &P, int slab, int slot, const RveDev *R, bool need_cold,
+                         double *lds)
 {
   constexpr int KS = vec_stride(K);
-  __shared__ double lds[4 * K * 16];
-  const int slab = blockIdx.x, tid = threadIdx.x;
-  const int slot = P.slab_rve[slab];
-  if (slot < 0) return;
-  RveDev *R = P.rves + slot;
-  if (R->phase != PH_RES) return;
+  const int tid = threadIdx.x;
   const int n = P.slab_node0[slab] + tid;
   const bool valid = n < R->nrow / 3;
   const long long rbase = R->row_base;
-  const int nslab = R->nslab;
-  double *part = P.part + (size_t)slab * kPart;
-  const bool need_cold = R->warm && R->sweep == 0;
   double v[4 * K];
 #pragma unroll
   for (int i = 0; i < 4 * K; i++) v[i] = 0.0;
@@ -616,61 +812,22 @@ __global__ __launch_bounds__(256) void k_step_res(EngPtrs P, SolveParams sp)
     }
   }
   block_sum<4 * K>(v, lds);
-  if (tid == 0)
-    for (int i = 0; i < 4 * K; i++) part[i] = v[i];
-  if (!arrive_last(&R->cntB, nslab)) return;
-  double s4[4 * K];
-  group_reduce<4 * K>(P.part + (size_t)R->first_slab * kPart, nslab, kPart, 0, s4, lds);
   if (tid == 0) {
-    unsigned done_out = 0;
-    for (int k = 0; k < K; k++) {
-      const double s[4] = {s4[k], s4[K + k], s4[2 * K + k], s4[3 * K + k]};
-      const double rn = sqrt(s[0]);
-      R->rnorm[k] = rn;
-      if (R->sweep == 0) R->r0[k] = need_cold ? sqrt(s[3]) : rn;
-      const bool conv = (k >= R->nload) || R->r0[k] == 0.0 || rn <= sp.rtol * R->r0[k] ||
-                        rn < sp.abs_tol || R->sweep >= sp.max_sweeps;
-      if (conv) done_out |= 1u << k;
-      // absolute target of this sweep: inner_rtol of the sweep's right-hand
-      // side, but no tighter than what the outer tolerance still needs
-      const double cap = (R->r0[k] > 0.0 && rn > 0.0) ? 0.5 * sp.rtol * R->r0[k] / rn : 0.0;
-      R->bnorm_in[k] = fmax(sp.inner_rtol, cap) * sqrt(s[1]);
-      R->beta[k] = 0.0;
-      R->rz[k] = s[2];
-      R->alpha[k] = 0.0;
-    }
-    const unsigned all = (1u << K) - 1u;
-    R->done_out = done_out;
-    if ((done_out & all) == all) {
-      R->phase = PH_AVG;
-    } else {
-      R->done_in = done_out;
-      R->its = 0;
-      R->phase = PH_INNER;
-    }
+    double *part = slab_part(P, slab);
+    for (int i = 0; i < 4 * K; i++) part[3 * K + i] = v[i];
   }
 }
 
-// ---------------------------------------------------------------------------
-// PH_AVG: volume averages of strain and stress (micro/src/homogenize.c:344-368)
-// in FP64, sum_gp (.) w_gp per element through sum_gp B w, then the tangent
-// (micro/src/homogenize.c:80).  One load case at a time keeps registers low.
 template <typename TS, int K>
-__global__ __launch_bounds__(256) void k_step_avg(EngPtrs P, SolveParams sp)
+__device__ void avg_slab(const EngPtrs &P, int slab, int slot, const RveDev *R, double *lds)
 {
   constexpr int KS = vec_stride(K);
-  __shared__ double lds[12 * 16];
   __shared__ double bw[144];
   __shared__ double cm[kNmat * 36];
-  __shared__ double sig[MM_MAX_RHS][6];
-  const int slab = blockIdx.x, tid = threadIdx.x;
-  const int slot = P.slab_rve[slab];
-  if (slot < 0) return;
-  RveDev *R = P.rves + slot;
-  if (R->phase != PH_AVG) return;
+  const int tid = threadIdx.x;
   const int nslab = R->nslab;
   const long long rbase = R->row_base;
-  double *part = P.part + (size_t)slab * kPart;
+  double *part = slab_part(P, slab);
   const int ls = slab - R->first_slab;
   const int nelm = R->nelm;
   const int epb = (nelm + nslab - 1) / nslab;
@@ -712,55 +869,26 @@ __global__ __launch_bounds__(256) void k_step_avg(EngPtrs P, SolveParams sp)
     if (tid == 0)
       for (int i = 0; i < 12; i++) part[k * 12 + i] = acc[i];
   }
-  if (!arrive_last(&R->cntA, nslab)) return;
-  __shared__ double tot[MM_MAX_RHS][12];
-#pragma unroll 1
-  for (int k = 0; k < K; k++) {
-    double s12[12];
-    group_reduce<12>(P.part + (size_t)R->first_slab * kPart, nslab, kPart, k * 12, s12, lds);
-    if (tid == 0)
-      for (int i = 0; i < 12; i++) tot[k][i] = s12[i];
+}
+
+template <typename TS, int K>
+__global__ __launch_bounds__(256) void k_step_r(EngPtrs P, SolveParams sp)
+{
+  __shared__ double lds[4 * K * 16];
+  const int slab = xcd_slab(), tid = threadIdx.x;
+  const int slot = P.slab_rve[slab];
+  if (slot < 0) return;
+  RveDev *R = P.rves + slot;
+  const SolveState *S = &R->st[1];
+  const int phase = S->phase;
+  if (slab == R->first_slab && tid == 0) {
+    SolveState N = *S;
+    if (phase == PH_RES) N.phase = PH_RESRED;
+    else if (phase == PH_AVG) N.phase = PH_AVGRED;
+    R->st[2] = N;
   }
-  if (tid == 0) {
-    RveOut *o = P.out + slot;
-    double rel = 0.0;
-    bool ok = true;
-#pragma unroll 1
-    for (int k = 0; k < K; k++) {
-#pragma unroll 1
-      for (int i = 0; i < 12; i++) {
-        const double s = tot[k][i] / R->vol;
-        if (i < 6) o->strain[k][i] = s;
-        else {
-          o->stress[k][i - 6] = s;
-          sig[k][i - 6] = s;
-        }
-      }
-      if (k < R->nload) {
-        const double r0 = R->r0[k], rn = R->rnorm[k];
-        rel = fmax(rel, (r0 > 0.0) ? rn / r0 : 0.0);
-        if (!(r0 == 0.0 || rn <= sp.rtol * r0 || rn < sp.abs_tol)) ok = false;
-      }
-    }
-    for (int i = 0; i < 36; i++) o->c[i] = 0.0;
-    if (R->tangent_mode == MM_TANGENT_UNIT && K >= 6) {
-      for (int i = 0; i < 6; i++)
-        for (int j = 0; j < 6; j++) o->c[j * 6 + i] = sig[i][j] / R->load[i][i];
-    } else if (R->tangent_mode == MM_TANGENT_FD && K >= 7) {
-      for (int i = 0; i < 6; i++)
-        for (int j = 0; j < 6; j++)
-          o->c[j * 6 + i] = (sig[i + 1][j] - sig[0][j]) / (R->load[i + 1][i] - R->load[0][i]);
-    }
-    o->residual_rel = rel;
-    o->its_total = R->its_total;
-    o->sweeps = R->sweep;
-    o->status = ok ? MM_OK : MM_NOT_CONVERGED;
-    R->status = o->status;
-    R->phase = PH_DONE;
-    __threadfence_system();
-    o->phase = PH_DONE;
-    __threadfence_system();
-  }
+  if (phase == PH_RES) res_slab<TS, K>(P, slab, slot, R, R->warm && S->sweep == 0, lds);
+  else if (phase == PH_AVG) avg_slab<TS, K>(P, slab, slot, R, lds);
 }
 
 // Stand-alone copy of the INNER-phase SpMV (w = A z) over every resident RVE,
@@ -769,7 +897,7 @@ template <typename TS, int K>
 __global__ __launch_bounds__(256) void k_spmv_time(EngPtrs P)
 {
   constexpr int KS = vec_stride(K);
-  const int slab = blockIdx.x, tid = threadIdx.x, SN = P.SN;
+  const int slab = xcd_slab(), tid = threadIdx.x, SN = P.SN;
   const int slot = P.slab_rve[slab];
   if (slot < 0) return;
   const RveDev *R = P.rves + slot;
@@ -793,12 +921,12 @@ __global__ __launch_bounds__(256) void k_spmv_time(EngPtrs P)
 template <typename TS, int K>
 static int launch_steps(const EngPtrs &P, const SolveParams &sp, int grid, int nsteps, hipStream_t s)
 {
+  if (grid % kXcd) return MM_ERR_ARG;  // the XCD-aware slab order needs whole groups of 8
   for (int i = 0; i < nsteps; i++) {
-    k_step_a<TS, K><<<grid, P.SN, 0, s>>>(P, sp);
+    k_step_a<TS, K><<<grid, P.SN, 3 * P.SN * vec_stride(K) * sizeof(TS), s>>>(P, sp);
     k_step_b<TS, K><<<grid, P.SN, 0, s>>>(P, sp);
-    k_step_res<TS, K><<<grid, P.SN, 0, s>>>(P, sp);
+    k_step_r<TS, K><<<grid, P.SN, 0, s>>>(P, sp);
   }
-  k_step_avg<TS, K><<<grid, P.SN, 0, s>>>(P, sp);
   MM_HIP(hipGetLastError());
   return 0;
 }

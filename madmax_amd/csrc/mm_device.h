@@ -44,17 +44,21 @@ __device__ inline void block_sum(double (&v)[NV], double *lds)
 // the block's partial results are stored.  Returns true in exactly one block,
 // the last to arrive, once all partials of the group are visible.  That block
 // resets the counter for the next launch.  No block waits on another.
+// Every block publishes with an agent-scope release only (an L2 write-back on
+// a multi-XCD part); the acquire, which invalidates the XCD's L2, is paid
+// once, by the last block.  A full __threadfence() in every block would
+// invalidate L2 thousands of times per launch.
 __device__ inline bool arrive_last(int *counter, int nblocks)
 {
   __shared__ int s_last;
   __syncthreads();
   if (threadIdx.x == 0) {
-    __threadfence();
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
     const int prev = atomicAdd(counter, 1);
     s_last = (prev == nblocks - 1);
     if (s_last) {
       atomicExch(counter, 0);
-      __threadfence();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
     }
   }
   __syncthreads();
@@ -74,6 +78,21 @@ __device__ inline void group_reduce(const double *p0, int nslab, int stride, int
   for (int b = threadIdx.x; b < nslab; b += blockDim.x)
 #pragma unroll
     for (int i = 0; i < NV; i++) out[i] += load_coherent(&p0[(size_t)b * stride + off + i]);
+  block_sum<NV>(out, lds);
+}
+
+// Like group_reduce, for partials written by an earlier launch (visible at
+// kernel start, so plain cached loads).  Every workgroup that calls it over
+// the same partials obtains the same bits.
+template <int NV>
+__device__ inline void group_sum(const double *p0, int nslab, int stride, int off, double (&out)[NV],
+                                 double *lds)
+{
+#pragma unroll
+  for (int i = 0; i < NV; i++) out[i] = 0.0;
+  for (int b = threadIdx.x; b < nslab; b += blockDim.x)
+#pragma unroll
+    for (int i = 0; i < NV; i++) out[i] += p0[(size_t)b * stride + off + i];
   block_sum<NV>(out, lds);
 }
 

@@ -63,11 +63,43 @@ __host__ __device__ inline void local_coords(int a, int *lx, int *ly, int *lz)
 }
 
 // ---------------------------------------------------------------------------
-// Solver phases of one resident RVE.  Kernel A handles INNER (SpMV + p.q)
-// and XUPD (x += d); kernel B handles RES (FP64 residual + start of a PCG
-// sweep) and INNER (vector updates + r.z); the averaging kernel handles AVG.
-enum Phase : int { PH_FREE = 0, PH_RES = 1, PH_INNER = 2, PH_XUPD = 3, PH_AVG = 4, PH_DONE = 5 };
+// Solver phases of one resident RVE.  One solver step is three launches,
+// A -> B -> R, over every resident slab:
+//   A  PH_RESRED: reduce the residual partials, start a PCG sweep or go to
+//                 PH_AVG; then as PH_INNER
+//      PH_INNER:  reduce r.r and r.z of the last B; converged -> x += d and
+//                 PH_RES; else SpMV, p, q and partial p.q
+//      PH_AVGRED: reduce the averaging partials, write the result, PH_DONE
+//   B  PH_INNER:  reduce p.q, alpha; d, r, z and partial r.r, r.z
+//   R  PH_RES:    FP64 residual, partial norms -> PH_RESRED
+//      PH_AVG:    partial volume averages -> PH_AVGRED
+// Every reduction is done by each workgroup of the consuming launch over the
+// producing launch's per-slab partials in slab order, so all workgroups of an
+// RVE agree bit for bit and no workgroup waits on, or fences for, another.
+enum Phase : int {
+  PH_FREE = 0,
+  PH_RES = 1,
+  PH_RESRED = 2,
+  PH_INNER = 3,
+  PH_AVG = 4,
+  PH_AVGRED = 5,
+  PH_DONE = 6
+};
 
+// Per-RVE solver scalars.  RveDev holds three copies: st[0] is written by A,
+// st[1] by B, st[2] by R (and by admission); each launch reads only its
+// predecessor's copy, and only the workgroup of the RVE's first slab writes.
+struct SolveState {
+  int phase, sweep, its, its_total;
+  int status;
+  unsigned done_in, done_out;
+  int pad_;
+  double beta[MM_MAX_RHS], rz[MM_MAX_RHS];
+  double bnorm_in[MM_MAX_RHS];  // absolute PCG target of the current sweep
+  double r0[MM_MAX_RHS], rnorm[MM_MAX_RHS];
+};
+
+// Admission-time description of a resident RVE plus its solver state.
 struct RveDev {
   // geometry and materials (written at admission)
   int nex, ney, nez, nx, ny, nz;
@@ -79,13 +111,7 @@ struct RveDev {
   double fx[MM_MAX_FIBERS], fy[MM_MAX_FIBERS], fr[MM_MAX_FIBERS];
   double young[kNmat], poisson[kNmat];
   double load[MM_MAX_RHS][MM_NVOI];
-  // solver state
-  int phase, sweep, its, its_total;
-  int cntA, cntB, status, pad_;
-  unsigned done_in, done_out;
-  double alpha[MM_MAX_RHS], beta[MM_MAX_RHS], rz[MM_MAX_RHS];
-  double bnorm_in[MM_MAX_RHS];  // absolute PCG target of the current sweep
-  double r0[MM_MAX_RHS], rnorm[MM_MAX_RHS];
+  SolveState st[3];
 };
 
 // Results written by the device into host-pinned memory.
