@@ -203,7 +203,7 @@ def main():
         "vs_baseline": None,
         "dtype": "f32 PCG in f64 refinement",
         "data": "synthetic (seeded two-phase fiber RVEs)",
-        "config": {"workload": "configs[1]: %d x %d^3 hex8 RVE tangents per step, 6-RHS block, "
+        "config": {"workload": "%d x %d^3 hex8 RVE tangents per step, 6-RHS block, "
                                "FP32 PCG + FP64 IR" % (B, n), "rve_elements": n ** 3,
                    "batch": B, "slab_nodes": args.slab, "inner_rtol": args.inner_rtol,
                    "rtol": args.rtol, "parallelism": "single GPU"},

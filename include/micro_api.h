@@ -130,6 +130,16 @@ int micro_get_displacement(double *x);
 #define MM_PREC_FP64 0  /* FP64 PCG */
 #define MM_PREC_MIXED 1 /* FP32 PCG inside FP64 iterative refinement */
 
+/* How the batched solver reads the ELL matrix of an RVE.  Both give the same
+ * SpMV bits.  FULL streams the per-row values and block columns from HBM (the
+ * reference's layout, slab-blocked).  DICT exploits the structured two-phase
+ * mesh: the 3-row block of an interior node is fixed by the materials of its
+ * 8 elements, so each RVE keeps a dictionary of the 256 possible blocks (the
+ * same FP64 sums, rounded) and one pattern byte per node; the matrix then
+ * lives in L1/L2 instead of HBM. */
+#define MM_ELL_FULL 0
+#define MM_ELL_DICT 1
+
 #define MM_TANGENT_NONE 0
 #define MM_TANGENT_UNIT 1 /* c[j*6+i] = sigma_i[j] / strain_i[i], loads i = 0..5 */
 #define MM_TANGENT_FD 2   /* c[j*6+i] = (sigma_{i+1}[j]-sigma_0[j]) / (eps_{i+1}[i]-eps_0[i]) */
@@ -150,6 +160,7 @@ typedef struct {
   int steps_per_launch;  /* PCG iterations per launch / graph replay */
   int use_graph;         /* capture the steps in a hipGraph */
   int sync_every_step;   /* plain path: host reads the state after every step */
+  int ell_storage;       /* MM_ELL_FULL or MM_ELL_DICT (default DICT) */
 } mm_engine_config;
 
 typedef struct {

@@ -151,6 +151,7 @@ struct mm_engine {
   mm_engine_config cfg;
   int SN = 128, K = 1, KS = 1;
   bool fp64 = false;
+  bool dict_mode = false;  // MM_ELL_DICT: no per-row ELL arrays, dictionary + pattern bytes
   size_t tsz = 4;
   hipStream_t stream = nullptr;
   // arena
@@ -164,6 +165,9 @@ struct mm_engine {
   int *slab_rve = nullptr, *slab_node0 = nullptr;
   RveDev *rves = nullptr;
   uint8_t *elem_mat = nullptr;
+  uint8_t *npat = nullptr;
+  void *dict = nullptr;      // solver dictionary (TS); aliases dict64 in FP64 mode
+  double *dict64 = nullptr;  // FP64 dictionary, the residual's matrix
   double *ke_tab = nullptr, *c_tab = nullptr, *bw_tab = nullptr;
   RveOut *out_host = nullptr, *out_dev = nullptr;
   size_t arena_bytes = 0;
@@ -222,6 +226,9 @@ struct mm_engine {
     P.c_tab = c_tab;
     P.bw_tab = bw_tab;
     P.elem_mat = elem_mat;
+    P.npat = npat;
+    P.dict = dict;
+    P.dict64 = dict64;
     P.SN = SN;
     return P;
   }
@@ -245,15 +252,17 @@ struct mm_engine {
   size_t bytes_per_slab() const
   {
     const size_t RS = (size_t)rows_per_slab();
-    return RS * kW * tsz + (size_t)SN * 27 * 4 + RS * (KS * (8 + 5 * tsz) + tsz) +
-           kPartDoubles * 8 + 8 + (size_t)SN + 1;
+    const size_t ell = dict_mode ? 0 : RS * kW * tsz + (size_t)SN * 27 * 4;
+    return ell + RS * (KS * (8 + 5 * tsz) + tsz) + kPartDoubles * 8 + 8 + (size_t)SN * 2 + 1;
   }
 };
 
 static int engine_free_arena(mm_engine *e)
 {
   void *ptrs[] = {e->bcols, e->vals, e->dinv, e->x64, e->d, e->r, e->z, e->p, e->q, e->part,
-                  e->slab_rve, e->slab_node0, e->rves, e->elem_mat, e->ke_tab, e->c_tab, e->bw_tab,
+                  e->slab_rve, e->slab_node0, e->rves, e->elem_mat, e->npat,
+                  (e->dict != (void *)e->dict64) ? e->dict : nullptr, e->dict64, e->ke_tab, e->c_tab,
+                  e->bw_tab,
                   e->dv_states, e->dv_slots, e->dv_ke, e->dv_slabs, e->dv_phase};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
@@ -281,6 +290,7 @@ extern "C" int mm_engine_config_default(mm_engine_config *c)
   c->steps_per_launch = 16;
   c->use_graph = 1;
   c->sync_every_step = 0;
+  c->ell_storage = MM_ELL_DICT;
   return 0;
 }
 
@@ -294,6 +304,7 @@ extern "C" int mm_engine_create(const mm_engine_config *cfg, mm_engine **out)
   if (cfg->slab_nodes != 0 && cfg->slab_nodes != 64 && cfg->slab_nodes != 128 && cfg->slab_nodes != 256)
     return MM_ERR_ARG;
   if (cfg->precision != MM_PREC_FP64 && cfg->precision != MM_PREC_MIXED) return MM_ERR_ARG;
+  if (cfg->ell_storage != MM_ELL_FULL && cfg->ell_storage != MM_ELL_DICT) return MM_ERR_ARG;
   if (!(cfg->rtol >= 0) || !(cfg->inner_rtol > 0) || cfg->max_inner_its < 1 || cfg->max_sweeps < 1)
     return MM_ERR_ARG;
   int ndev = 0;
@@ -315,6 +326,7 @@ extern "C" int mm_engine_create(const mm_engine_config *cfg, mm_engine **out)
   e->KS = vec_stride_host(e->K);
   e->fp64 = (cfg->precision == MM_PREC_FP64);
   e->tsz = e->fp64 ? 8 : 4;
+  e->dict_mode = (cfg->ell_storage == MM_ELL_DICT);
   e->rve_cap = cfg->max_rves > 0 ? cfg->max_rves : 4096;
   e->t0 = std::chrono::steady_clock::now();
   int rc = 0;
@@ -327,7 +339,9 @@ extern "C" int mm_engine_create(const mm_engine_config *cfg, mm_engine **out)
     }
     budget = (size_t)(0.8 * (double)fr);
   }
-  const size_t per_rve = (size_t)kNmat * (kKe + 36) * 8 + 144 * 8 + sizeof(RveDev) + sizeof(RveOut);
+  const size_t per_rve = (size_t)kNmat * (kKe + 36) * 8 + 144 * 8 + sizeof(RveDev) + sizeof(RveOut) +
+                         (size_t)kDictPat * kDictEntry * 8 +
+                         ((e->dict_mode && !e->fp64) ? (size_t)kDictPat * kDictEntry * 4 : 0);
   if (budget <= (size_t)e->rve_cap * per_rve + e->bytes_per_slab()) {
     delete e;
     return MM_ERR_ALLOC;
@@ -355,8 +369,16 @@ extern "C" int mm_engine_create(const mm_engine_config *cfg, mm_engine **out)
     }                                                                       \
     e->arena_bytes += (bytes);                                              \
   } while (0)
-  EALLOC(e->bcols, (size_t)e->nslab_cap * e->SN * 27 * 4);
-  EALLOC(e->vals, slots * e->tsz);
+  if (!e->dict_mode) {
+    EALLOC(e->bcols, (size_t)e->nslab_cap * e->SN * 27 * 4);
+    EALLOC(e->vals, slots * e->tsz);
+  }
+  EALLOC(e->dict64, (size_t)e->rve_cap * kDictPat * kDictEntry * 8);
+  if (e->dict_mode) {
+    if (e->fp64) e->dict = e->dict64;
+    else EALLOC(e->dict, (size_t)e->rve_cap * kDictPat * kDictEntry * 4);
+  }
+  EALLOC(e->npat, (size_t)e->nslab_cap * e->SN);
   EALLOC(e->dinv, nrows * e->tsz);
   EALLOC(e->x64, nrows * e->KS * 8);
   EALLOC(e->d, nrows * e->KS * e->tsz);
@@ -636,8 +658,14 @@ static int admit(mm_engine *e)
   if ((rc = eng_launch_ke((int)nke, e->dv_ke, e->rves, e->ke_tab, e->c_tab, e->bw_tab, e->stream)))
     return rc;
   if ((rc = eng_launch_phase((int)nph, e->dv_phase, e->rves, e->elem_mat, e->stream))) return rc;
-  if ((rc = eng_launch_assemble(e->fp64, e->K, P, (int)nsl, e->dv_slabs, 0, e->stream))) return rc;
-  if ((rc = eng_launch_assemble(e->fp64, e->K, P, (int)nsl_re, e->dv_slabs + nsl, 1, e->stream)))
+  if ((rc = eng_launch_dict(true, (int)fresh.size(), e->dv_slots, e->ke_tab, e->dict64, e->stream))) return rc;
+  if (e->dict_mode && !e->fp64 &&
+      (rc = eng_launch_dict(false, (int)fresh.size(), e->dv_slots, e->ke_tab, e->dict, e->stream)))
+    return rc;
+  if ((rc = eng_launch_assemble(e->fp64, e->K, P, (int)nsl, e->dv_slabs, e->dict_mode ? ASM_DICT : ASM_FULL,
+                                e->stream)))
+    return rc;
+  if ((rc = eng_launch_assemble(e->fp64, e->K, P, (int)nsl_re, e->dv_slabs + nsl, ASM_X_ONLY, e->stream)))
     return rc;
   e->n_active += (int)n;
   return 0;
@@ -668,7 +696,7 @@ static int run_steps(mm_engine *e)
       e->gexec = nullptr;
       e->graph = nullptr;
       MM_HIP(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
-      int rc = eng_launch_steps(e->fp64, e->K, P, sp, grid, T, e->stream);
+      int rc = eng_launch_steps(e->fp64, e->K, e->dict_mode, P, sp, grid, T, e->stream);
       hipGraph_t g = nullptr;
       hipError_t ce = hipStreamEndCapture(e->stream, &g);
       if (rc) return rc;
@@ -679,7 +707,7 @@ static int run_steps(mm_engine *e)
     }
     MM_HIP(hipGraphLaunch(e->gexec, e->stream));
   } else {
-    int rc = eng_launch_steps(e->fp64, e->K, P, sp, grid, T, e->stream);
+    int rc = eng_launch_steps(e->fp64, e->K, e->dict_mode, P, sp, grid, T, e->stream);
     if (rc) return rc;
   }
   return 0;
@@ -871,6 +899,54 @@ extern "C" int mm_engine_rve_sizes(mm_engine *e, uint64_t key, int *nrow, int *n
   return 0;
 }
 
+// Expands dictionary storage into the reference's row-major ELL: a boundary
+// node's rows are the identity over its class's slots, an interior node's
+// rows are its dictionary entry at the structured neighbour columns.
+static int download_ell_dict(mm_engine *e, int s, int *cols, double *vals)
+{
+  const HostSlot &h = e->slots[s];
+  const size_t ent = (size_t)kDictPat * kDictEntry;
+  std::vector<unsigned char> dict(ent * e->tsz);
+  std::vector<uint8_t> pat((size_t)h.nslab * e->SN);
+  MM_HIP(hipMemcpyAsync(dict.data(), (char *)e->dict + (size_t)s * ent * e->tsz, ent * e->tsz,
+                        hipMemcpyDeviceToHost, e->stream));
+  MM_HIP(hipMemcpyAsync(pat.data(), e->npat + (size_t)h.first_slab * e->SN, pat.size(),
+                        hipMemcpyDeviceToHost, e->stream));
+  MM_HIP(hipStreamSynchronize(e->stream));
+  SlotTables T;
+  build_slot_tables(&T);
+  const mm_rve_spec &rs = h.req.rve;
+  const int nx = rs.nex + 1, ny = rs.ney + 1, nz = rs.nez + 1, nxy = nx * ny;
+  auto cls_of = [](int i, int n) { return i == 0 ? 0 : (i == n - 1 ? 2 : 1); };
+  for (int n = 0; n < h.nrow / 3; n++) {
+    const int ix = n % nx, iy = (n / nx) % ny, iz = n / nxy;
+    const int cls = cls_of(ix, nx) + 3 * cls_of(iy, ny) + 9 * cls_of(iz, nz);
+    const int ns = T.nslot[cls];
+    for (int d = 0; d < 3; d++) {
+      const size_t row = 3 * (size_t)n + d;
+      for (int j = 0; j < kW; j++) {
+        const int t = j / 3, dd = j % 3;
+        int c = -1;
+        double v = 0.0;
+        if (t < ns) {
+          const int key = T.off[cls][t];
+          const int m = n + (key % 3 - 1) + ((key / 3) % 3 - 1) * nx + (key / 9 - 1) * nxy;
+          c = 3 * m + dd;
+          if (cls != 13) {
+            v = (m == n && dd == d) ? 1.0 : 0.0;
+          } else {
+            const size_t src = (size_t)pat[n] * kDictEntry + (size_t)t * kDictBlk + dd * 4 + d;
+            v = e->fp64 ? ((const double *)dict.data())[src] : (double)((const float *)dict.data())[src];
+          }
+        }
+        cols[row * kW + j] = c;
+        vals[row * kW + j] = v;
+      }
+    }
+  }
+  return 0;
+}
+
 extern "C" int mm_engine_download_ell(mm_engine *e, uint64_t key, int *cols, double *vals)
 {
   if (!e || !cols || !vals) return MM_ERR_ARG;
@@ -878,6 +954,7 @@ extern "C" int mm_engine_download_ell(mm_engine *e, uint64_t key, int *cols, dou
   const int s = find_cached(e, key);
   if (s < 0) return MM_ERR_STATE;
   const HostSlot &h = e->slots[s];
+  if (e->dict_mode) return download_ell_dict(e, s, cols, vals);
   const size_t RS = (size_t)e->rows_per_slab();
   const size_t nv = (size_t)h.nslab * RS * kW, nc = (size_t)h.nslab * e->SN * 27;
   std::vector<int> hc(nc);
@@ -941,20 +1018,24 @@ extern "C" int mm_engine_time_spmv(mm_engine *e, int reps, double *avg_ms, doubl
   if (hw <= 0 || e->n_active > 0) return MM_ERR_STATE;  // it overwrites q
   const int grid = solver_grid(e, hw);
   EngPtrs P = e->ptrs();
-  // algorithmic bytes of one launch: every resident row reads its 81 values,
-  // every node its 27 block columns, and the vector z is read and w written
-  // once per row and load case
-  double rows = 0.0;
+  // algorithmic bytes of one launch: z read and w written once per row and
+  // load case; full storage: every row's 81 values and every node's 27 block
+  // columns; dictionary storage: one pattern byte per node and each resident
+  // RVE's dictionary once
+  double rows = 0.0, nrve = 0.0;
   for (int s = 0; s < e->rve_cap; s++)
-    if (e->slots[s].state != SL_FREE) rows += e->slots[s].nrow;
+    if (e->slots[s].state != SL_FREE) {
+      rows += e->slots[s].nrow;
+      nrve += 1.0;
+    }
   hipEvent_t a, b;
   MM_HIP(hipEventCreate(&a));
   MM_HIP(hipEventCreate(&b));
-  int rc = eng_launch_time(e->fp64, e->K, P, grid, e->stream);  // warm-up
+  int rc = eng_launch_time(e->fp64, e->K, e->dict_mode, P, grid, e->stream);  // warm-up
   if (rc) return rc;
   MM_HIP(hipEventRecord(a, e->stream));
   for (int i = 0; i < reps; i++) {
-    rc = eng_launch_time(e->fp64, e->K, P, grid, e->stream);
+    rc = eng_launch_time(e->fp64, e->K, e->dict_mode, P, grid, e->stream);
     if (rc) return rc;
   }
   MM_HIP(hipEventRecord(b, e->stream));
@@ -965,7 +1046,11 @@ extern "C" int mm_engine_time_spmv(mm_engine *e, int reps, double *avg_ms, doubl
   (void)hipEventDestroy(b);
   e->spmv_ms = ms / reps;
   if (avg_ms) *avg_ms = e->spmv_ms;
-  if (bytes) *bytes = rows * (kW * (double)e->tsz + 27.0 * 4.0 / 3.0 + 2.0 * e->K * e->tsz);
+  if (bytes) {
+    const double vec = rows * 2.0 * e->K * e->tsz;
+    *bytes = e->dict_mode ? vec + rows / 3.0 + nrve * kDictPat * kDictEntry * (double)e->tsz
+                          : vec + rows * (kW * (double)e->tsz + 27.0 * 4.0 / 3.0);
+  }
   return 0;
 }
 

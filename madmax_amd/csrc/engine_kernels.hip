@@ -207,7 +207,7 @@ __global__ void k_assemble(int nblk, const int *__restrict__ slab_list, const in
                            const int *__restrict__ slab_node0, const RveDev *__restrict__ rves,
                            const double *__restrict__ ke_tab, const uint8_t *__restrict__ elem_mat,
                            int *__restrict__ bcols, TV *__restrict__ vals, TV *__restrict__ dinv,
-                           double *__restrict__ x64, int SN, int mode)
+                           uint8_t *__restrict__ npat, double *__restrict__ x64, int SN, int mode)
 {
 #pragma clang fp contract(off)
   constexpr int KS = vec_stride(K);
@@ -223,10 +223,13 @@ __global__ void k_assemble(int nblk, const int *__restrict__ slab_list, const in
   const long long grow0 = R.row_base + 3LL * n;
   const int nn = R.nrow / 3;
   if (n >= nn) {
-    if (mode == 0) {
+    if (mode == ASM_FULL) {
       for (int t = 0; t < 27; t++) bcols[cb + (long long)t * SN + tid] = -1;
       for (int j = 0; j < kW; j++)
         for (int d = 0; d < 3; d++) vals[vb + (long long)j * RS + 3 * tid + d] = (TV)0;
+    }
+    if (mode != ASM_X_ONLY) {
+      npat[(long long)slab * SN + tid] = 0;
       for (int d = 0; d < 3; d++) dinv[grow0 + d] = (TV)0;
     }
     for (int d = 0; d < 3; d++)
@@ -237,7 +240,37 @@ __global__ void k_assemble(int nblk, const int *__restrict__ slab_list, const in
   const int ix = n % nx, iy = (n / nx) % R.ny, iz = n / nxy;
   const int cls = node_class(ix, iy, iz, nx, R.ny, R.nz);
   const bool bc = (cls != 13);
-  if (mode == 0) {
+  if (mode != ASM_X_ONLY) {
+    // material pattern of the 8 elements around an interior node: bit c is
+    // the phase of the element at offset ((c&1)-1, ((c>>1)&1)-1, (c>>2)-1)
+    int pat = 0;
+    if (!bc)
+      for (int c = 0; c < 8; c++) {
+        const int e = (ix + (c & 1) - 1) + (iy + ((c >> 1) & 1) - 1) * R.nex +
+                      (iz + (c >> 2) - 1) * R.nex * R.ney;
+        pat |= (int)elem_mat[R.elem_base + e] << c;
+      }
+    npat[(long long)slab * SN + tid] = (uint8_t)pat;
+  }
+  if (mode == ASM_DICT) {
+    // only the diagonal, summed exactly as the full assembly sums it
+    double diag[3] = {1.0, 1.0, 1.0};
+    if (!bc) {
+      const double *ke0 = ke_tab + (size_t)slot * kNmat * kKe;
+      const int t = g_slots.diag_slot[13];
+      const int nc = g_slots.ncontrib[13][t];
+      for (int c = 0; c < nc; c++) {
+        const int code = g_slots.contrib[13][t][c];
+        const int ox = (code & 1) - 1, oy = ((code >> 1) & 1) - 1, oz = ((code >> 2) & 1) - 1;
+        const int a = (code >> 3) & 7, b = (code >> 6) & 7;
+        const int e = (ix + ox) + (iy + oy) * R.nex + (iz + oz) * R.nex * R.ney;
+        const double *ke = ke0 + (size_t)elem_mat[R.elem_base + e] * kKe + 3 * a * kNd + 3 * b;
+        for (int d = 0; d < 3; d++) diag[d] = (c == 0) ? ke[d * kNd + d] : diag[d] + ke[d * kNd + d];
+      }
+    }
+    for (int d = 0; d < 3; d++) dinv[grow0 + d] = (TV)(1.0 / diag[d]);
+  }
+  if (mode == ASM_FULL) {
     const int ns = g_slots.nslot[cls];
     const int dslot = g_slots.diag_slot[cls];
     const double *ke0 = ke_tab + (size_t)slot * kNmat * kKe;
@@ -291,6 +324,38 @@ __global__ void k_assemble(int nblk, const int *__restrict__ slab_list, const in
         x64[(grow0 + d) * KS + k] = 0.0;
       }
     }
+}
+
+// Dictionary of interior ELL blocks per RVE: entry (slot, pattern p) is the
+// 3-row block of an interior node whose element at offset code c has phase
+// bit c of p.  Each value is the FP64 sum over the contributing elements in
+// the order the full assembly (and micro/src/assembly.c:97-109 with
+// src/ell.c:46-84) adds them, then rounded to TV, so a dictionary row equals
+// the assembled row bit for bit.  Grid: n * 256 workgroups, 324 lanes used.
+template <typename TV>
+__global__ __launch_bounds__(384) void k_dict(int n, const int *__restrict__ slots,
+                                              const double *__restrict__ ke_tab, TV *__restrict__ dict)
+{
+#pragma clang fp contract(off)
+  const int i = blockIdx.x / kDictPat, pat = blockIdx.x % kDictPat;
+  if (i >= n) return;
+  const int tid = threadIdx.x;
+  if (tid >= kDictEntry) return;
+  const int slot = slots[i];
+  const int t = tid / kDictBlk, dd = (tid % kDictBlk) / 4, d = tid % 4;
+  double v = 0.0;
+  if (d < 3) {
+    const double *ke0 = ke_tab + (size_t)slot * kNmat * kKe;
+    const int nc = g_slots.ncontrib[13][t];
+    for (int c = 0; c < nc; c++) {
+      const int code = g_slots.contrib[13][t][c];
+      const int a = (code >> 3) & 7, b = (code >> 6) & 7;
+      const int mat = (pat >> (code & 7)) & 1;
+      const double kv = ke0[(size_t)mat * kKe + (3 * a + d) * kNd + 3 * b + dd];
+      v = (c == 0) ? kv : v + kv;
+    }
+  }
+  dict[((size_t)slot * kDictPat + pat) * kDictEntry + tid] = (TV)v;
 }
 
 template <typename T>
@@ -366,28 +431,100 @@ __device__ __forceinline__ void node_spmv(const int *__restrict__ bcols, const T
   }
 }
 
+// The same product from the dictionary: an interior node's 27 blocks come
+// from entry npat of its RVE (lanes of one phase read the same addresses, so
+// the values stay in L1/L2) at the structured neighbour offsets; a boundary
+// node's row is the identity, whose full-ELL product z + 0 is reproduced
+// exactly.  Same fma order as node_spmv, so the same bits.
+template <typename TS, int KS>
+__device__ __forceinline__ void node_spmv_dict(const TS *__restrict__ dict_rve, int pat, bool bc, int n,
+                                               int nx, int nxy, long long rbase,
+                                               const TS *__restrict__ z, TS (&w)[3][KS])
+{
+  if (bc) {
+    TS xz[3 * KS];
+    load_vec<TS, 3 * KS>(z + (rbase + 3LL * n) * KS, xz);
+#pragma unroll
+    for (int d = 0; d < 3; d++)
+#pragma unroll
+      for (int k = 0; k < KS; k++) w[d][k] = xz[d * KS + k] + (TS)0;
+    return;
+  }
+#pragma unroll
+  for (int d = 0; d < 3; d++)
+#pragma unroll
+    for (int k = 0; k < KS; k++) w[d][k] = (TS)0;
+  const TS *D = dict_rve + (size_t)pat * kDictEntry;
+#pragma unroll 3
+  for (int t = 0; t < 27; t++) {
+    const int key = g_slots.off[13][t];
+    const int m = n + (key % 3 - 1) + ((key / 3) % 3 - 1) * nx + (key / 9 - 1) * nxy;
+    TS v[3][4];  // [d'][d]
+#pragma unroll
+    for (int dd = 0; dd < 3; dd++) load_vec<TS, 4>(D + t * kDictBlk + dd * 4, v[dd]);
+    TS xz[3 * KS];
+    load_vec<TS, 3 * KS>(z + (rbase + 3LL * m) * KS, xz);
+#pragma unroll
+    for (int dd = 0; dd < 3; dd++)
+#pragma unroll
+      for (int d = 0; d < 3; d++)
+#pragma unroll
+        for (int k = 0; k < KS; k++) w[d][k] = fma(v[dd][d], xz[dd * KS + k], w[d][k]);
+  }
+}
+
+// Product w = A z for the node of lane tid of a slab, from either storage.
+template <typename TS, int KS, bool DICT>
+__device__ __forceinline__ void slab_node_spmv(const EngPtrs &P, const RveDev *R, int slot, int slab,
+                                               int tid, int n, TS (&w)[3][KS])
+{
+  const int SN = P.SN;
+  if (DICT) {
+    const int nx = R->nx, nxy = R->nx * R->ny;
+    const int ix = n % nx, iy = (n / nx) % R->ny, iz = n / nxy;
+    const bool bc = node_class(ix, iy, iz, nx, R->ny, R->nz) != 13;
+    const TS *dict_rve = (const TS *)P.dict + (size_t)slot * kDictPat * kDictEntry;
+    node_spmv_dict<TS, KS>(dict_rve, P.npat[(long long)slab * SN + tid], bc, n, nx, nxy, R->row_base,
+                           (const TS *)P.z, w);
+  } else {
+    node_spmv<TS, KS>(P.bcols, (const TS *)P.vals, (long long)slab * 27 * SN,
+                      (long long)slab * kW * 3 * SN, SN, tid, R->row_base, (const TS *)P.z, w);
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Element-wise passes over a slab's vectors.  A slab's rows are contiguous in
-// every [row][KS] vector, so lane g of the workgroup handles VEC consecutive
-// values g*VEC.. of the slab (VEC = 2 for even KS): each vector access of a
-// wavefront is one contiguous 512-byte (FP32) run.  The load case of value
-// i is i % KS; per-load-case scalars come from LDS.
+// every [row][KS] vector, so a lane handles VEC consecutive values (VEC = 2
+// for even KS) and a wavefront's access is one contiguous run.  The lanes
+// stride by S, the largest count <= blockDim with S * VEC a multiple of KS,
+// so lane tid always meets the same load cases k0 = (tid * VEC) % KS ..
+// k0 + VEC - 1: their scalars stay in registers and the dot products
+// accumulate in VEC plain scalars (a per-value case index would put the
+// accumulators in scratch memory).
 template <int KS>
 struct SlabVec {
   static constexpr int VEC = (KS % 2 == 0) ? 2 : 1;
 };
 
-// Adds a[j] to acc[k0 + j] for the (VEC) load cases k0.. without dynamic
-// register indexing.
-template <int K, int KS, int VEC>
-__device__ __forceinline__ void acc_by_case(double (&acc)[K], int k0, const double (&a)[VEC])
+template <int KS, int VEC>
+__device__ __forceinline__ int fixed_case_stride()
+{
+  int S = blockDim.x;
+  while ((S * VEC) % KS) S--;
+  return S;
+}
+
+// v[k] = the lane's accumulator of load case k (zero for cases it never met).
+template <int K, int VEC>
+__device__ __forceinline__ void route_cases(const double (&a)[VEC], int k0, double (&v)[K])
 {
 #pragma unroll
-  for (int kk = 0; kk < KS; kk += VEC)
-    if (kk == k0)
+  for (int k = 0; k < K; k++) {
+    double s = 0.0;
 #pragma unroll
-      for (int j = 0; j < VEC; j++)
-        if (kk + j < K) acc[(kk + j < K) ? kk + j : 0] += a[j];
+    for (int j = 0; j < VEC; j++) s = (k == k0 + j) ? a[j] : s;
+    v[k] = s;
+  }
 }
 
 // Per-slab partial sums of the launch that produced them, summed over the
@@ -484,9 +621,9 @@ __device__ void finish_rve(const EngPtrs &P, int slot, const RveDev *R, SolveSta
     if (threadIdx.x == 0)
       for (int i = 0; i < 12; i++) tot[k][i] = s12[i];
   }
+  __shared__ double sig[MM_MAX_RHS][6];
   if (threadIdx.x != 0) return;
   RveOut *o = P.out + slot;
-  double sig[MM_MAX_RHS][6];
   double rel = 0.0;
   bool ok = true;
 #pragma unroll 1
@@ -534,7 +671,7 @@ __device__ void finish_rve(const EngPtrs &P, int slot, const RveDev *R, SolveSta
 // node, staged in LDS), p = z + beta p, q = w + beta q (so q = A p without
 // gathering p) and the partial p.q.  PH_AVGRED: the first slab's workgroup
 // writes the result.
-template <typename TS, int K>
+template <typename TS, int K, bool DICT>
 __global__ __launch_bounds__(256) void k_step_a(EngPtrs P, SolveParams sp)
 {
   constexpr int KS = vec_stride(K);
@@ -595,11 +732,12 @@ __global__ __launch_bounds__(256) void k_step_a(EngPtrs P, SolveParams sp)
 
   // PH_INNER
   if (tid < KS) s_beta[tid] = (tid < K) ? (TS)N.beta[tid] : (TS)0;
+  const int S = fixed_case_stride<KS, VEC>();
+  const int k0 = (tid * VEC) % KS;
   const int n = n0 + tid;
   if (n < R->nrow / 3) {
     TS w[3][KS];
-    node_spmv<TS, KS>(P.bcols, (const TS *)P.vals, (long long)slab * 27 * SN,
-                      (long long)slab * kW * 3 * SN, SN, tid, rbase, (const TS *)P.z, w);
+    slab_node_spmv<TS, KS, DICT>(P, R, slot, slab, tid, n, w);
 #pragma unroll
     for (int d = 0; d < 3; d++)
 #pragma unroll
@@ -609,28 +747,30 @@ __global__ __launch_bounds__(256) void k_step_a(EngPtrs P, SolveParams sp)
   TS *p = (TS *)P.p;
   TS *q = (TS *)P.q;
   const TS *z = (const TS *)P.z;
-  double v[K];
+  TS bj[VEC];
+  double acc[VEC];
 #pragma unroll
-  for (int k = 0; k < K; k++) v[k] = 0.0;
-  for (int g = tid; g < ngroups; g += blockDim.x) {
+  for (int j = 0; j < VEC; j++) {
+    bj[j] = s_beta[k0 + j];
+    acc[j] = 0.0;
+  }
+  for (int g = tid; tid < S && g < ngroups; g += S) {
     const long long i0 = f0 + (long long)g * VEC;
-    const int k0 = (g * VEC) % KS;
     TS zi[VEC], pi[VEC], qi[VEC];
     load_vec<TS, VEC>(z + i0, zi);
     load_vec<TS, VEC>(p + i0, pi);
     load_vec<TS, VEC>(q + i0, qi);
-    double a[VEC];
 #pragma unroll
     for (int j = 0; j < VEC; j++) {
-      const TS b = s_beta[k0 + j];
-      pi[j] = fma(b, pi[j], zi[j]);
-      qi[j] = fma(b, qi[j], ws[g * VEC + j]);
-      a[j] = (double)pi[j] * (double)qi[j];
+      pi[j] = fma(bj[j], pi[j], zi[j]);
+      qi[j] = fma(bj[j], qi[j], ws[g * VEC + j]);
+      acc[j] += (double)pi[j] * (double)qi[j];
     }
     store_vec<TS, VEC>(p + i0, pi);
     store_vec<TS, VEC>(q + i0, qi);
-    acc_by_case<K, KS, VEC>(v, k0, a);
   }
+  double v[K];
+  route_cases<K, VEC>(acc, k0, v);
   block_sum<K>(v, lds);
   if (tid == 0) {
     double *part = slab_part(P, slab);
@@ -674,37 +814,39 @@ __global__ __launch_bounds__(256) void k_step_b(EngPtrs P, SolveParams sp)
   const int ngroups = rows * KS / VEC;
   TS *dv = (TS *)P.d, *rv = (TS *)P.r, *zv = (TS *)P.z;
   const TS *p = (const TS *)P.p, *q = (const TS *)P.q, *dinv = (const TS *)P.dinv;
-  double v[K], u[K];
+  const int LS = fixed_case_stride<KS, VEC>();
+  const int k0 = (tid * VEC) % KS;
+  TS aj[VEC];
+  double arr[VEC], arz[VEC];
 #pragma unroll
-  for (int k = 0; k < K; k++) {
-    v[k] = 0.0;
-    u[k] = 0.0;
+  for (int j = 0; j < VEC; j++) {
+    aj[j] = s_alpha[k0 + j];
+    arr[j] = 0.0;
+    arz[j] = 0.0;
   }
-  for (int g = tid; g < ngroups; g += blockDim.x) {
+  for (int g = tid; tid < LS && g < ngroups; g += LS) {
     const long long i0 = f0 + (long long)g * VEC;
-    const int k0 = (g * VEC) % KS;
     const TS dg = dinv[row0 + (g * VEC) / KS];
     TS di[VEC], ri[VEC], pi[VEC], qi[VEC], zi[VEC];
     load_vec<TS, VEC>(dv + i0, di);
     load_vec<TS, VEC>(rv + i0, ri);
     load_vec<TS, VEC>(p + i0, pi);
     load_vec<TS, VEC>(q + i0, qi);
-    double arr[VEC], arz[VEC];
 #pragma unroll
     for (int j = 0; j < VEC; j++) {
-      const TS a = s_alpha[k0 + j];
-      di[j] = fma(a, pi[j], di[j]);
-      ri[j] = fma(-a, qi[j], ri[j]);
+      di[j] = fma(aj[j], pi[j], di[j]);
+      ri[j] = fma(-aj[j], qi[j], ri[j]);
       zi[j] = dg * ri[j];
-      arr[j] = (double)ri[j] * (double)ri[j];
-      arz[j] = (double)ri[j] * (double)zi[j];
+      arr[j] += (double)ri[j] * (double)ri[j];
+      arz[j] += (double)ri[j] * (double)zi[j];
     }
     store_vec<TS, VEC>(dv + i0, di);
     store_vec<TS, VEC>(rv + i0, ri);
     store_vec<TS, VEC>(zv + i0, zi);
-    acc_by_case<K, KS, VEC>(v, k0, arr);
-    acc_by_case<K, KS, VEC>(u, k0, arz);
   }
+  double v[K], u[K];
+  route_cases<K, VEC>(arr, k0, v);
+  route_cases<K, VEC>(arz, k0, u);
   double vu[2 * K];
 #pragma unroll
   for (int k = 0; k < K; k++) {
@@ -720,8 +862,8 @@ __global__ __launch_bounds__(256) void k_step_b(EngPtrs P, SolveParams sp)
 
 // ---------------------------------------------------------------------------
 // Kernel R (reads st[1], writes st[2]).
-// PH_RES: the FP64 residual r = -K x64, computed matrix-free from the element
-// stiffness tables (the Newton / refinement residual of
+// PH_RES: the FP64 residual r = -K x64 over the FP64 dictionary rows (the
+// assembled matrix; the Newton / refinement residual of
 // micro/src/assembly.c:33-82; boundary rows keep a zero residual, :59-74),
 // and the start of the next PCG sweep (d = 0, z = dinv r, p = q = 0): every
 // refinement sweep solves the correction equation from zero.  The relative
@@ -743,57 +885,60 @@ __device__ void res_slab(const EngPtrs &P, int slab, int slot, const RveDev *R, 
 #pragma unroll
   for (int i = 0; i < 4 * K; i++) v[i] = 0.0;
   if (valid) {
-    TS *dv = (TS *)P.d, *rv = (TS *)P.r, *zv = (TS *)P.z;
-    const TS *dinv = (const TS *)P.dinv;
-    const int nx = R->nx, nxy = R->nx * R->ny;
-    const int ix = n % nx, iy = (n / nx) % R->ny, iz = n / nxy;
-    const int cls = node_class(ix, iy, iz, nx, R->ny, R->nz);
-    const double *ke0 = P.ke_tab + (size_t)slot * kNmat * kKe;
-#pragma unroll 1
-    for (int d = 0; d < 3; d++) {
-      const long long grow = rbase + 3LL * n + d;
-      double r64[K], rc64[K];
+    const int nx = R->nx, ny = R->ny, nz = R->nz, nxy = nx * ny;
+    const int ix = n % nx, iy = (n / nx) % ny, iz = n / nxy;
+    const int cls = node_class(ix, iy, iz, nx, ny, nz);
+    // K x64 over the node's 27 FP64 dictionary blocks (the assembled FP64
+    // rows); boundary rows keep a zero residual
+    double r64[3][K], rc64[3][K];
+#pragma unroll
+    for (int d = 0; d < 3; d++)
 #pragma unroll
       for (int k = 0; k < K; k++) {
-        r64[k] = 0.0;
-        rc64[k] = 0.0;
+        r64[d][k] = 0.0;
+        rc64[d][k] = 0.0;
       }
-      if (cls == 13) {
-        const int ne = g_slots.nelem[cls];
+    if (cls == 13) {
+      const double *D = P.dict64 + ((size_t)slot * kDictPat + P.npat[(long long)slab * P.SN + tid]) *
+                                       kDictEntry;
 #pragma unroll 1
-        for (int i = 0; i < ne; i++) {
-          const int code = g_slots.elem[cls][i];
-          const int ox = (code & 1) - 1, oy = ((code >> 1) & 1) - 1, oz = ((code >> 2) & 1) - 1;
-          const int a = (code >> 3) & 7;
-          const int e = (ix + ox) + (iy + oy) * R->nex + (iz + oz) * R->nex * R->ney;
-          const double *ke = ke0 + (size_t)P.elem_mat[R->elem_base + e] * kKe + (3 * a + d) * kNd;
-          const int n0 = n + ox + oy * nx + oz * nxy;
-#pragma unroll 2
-          for (int b = 0; b < 8; b++) {
-            int lx, ly, lz;
-            local_coords(b, &lx, &ly, &lz);
-            const long long mrow = rbase + 3LL * (n0 + lx + ly * nx + lz * nxy);
-            const int jx = ix + ox + lx, jy = iy + oy + ly, jz = iz + oz + lz;
-            const bool nb_bc = jx == 0 || jx == nx - 1 || jy == 0 || jy == R->ny - 1 || jz == 0 ||
-                               jz == R->nz - 1;
-            for (int dd = 0; dd < 3; dd++) {
-              const double kv = ke[3 * b + dd];
-              const double *xv = P.x64 + (mrow + dd) * KS;
+      for (int t = 0; t < 27; t++) {
+        const int key = g_slots.off[13][t];
+        const int dx = key % 3 - 1, dy = (key / 3) % 3 - 1, dz = key / 9 - 1;
+        const int m = n + dx + dy * nx + dz * nxy;
+        const int jx = ix + dx, jy = iy + dy, jz = iz + dz;
+        const bool nb_bc = jx == 0 || jx == nx - 1 || jy == 0 || jy == ny - 1 || jz == 0 || jz == nz - 1;
+        double vb[3][4];
 #pragma unroll
-              for (int k = 0; k < K; k++) r64[k] = fma(kv, xv[k], r64[k]);
-              if (need_cold && nb_bc)
+        for (int dd = 0; dd < 3; dd++) load_vec<double, 4>(D + t * kDictBlk + dd * 4, vb[dd]);
+        const double *xm = P.x64 + (rbase + 3LL * m) * KS;
 #pragma unroll
-                for (int k = 0; k < K; k++) rc64[k] = fma(kv, xv[k], rc64[k]);
-            }
-          }
+        for (int dd = 0; dd < 3; dd++) {
+          double xv[KS];
+          load_vec<double, KS>(xm + dd * KS, xv);
+#pragma unroll
+          for (int d = 0; d < 3; d++)
+#pragma unroll
+            for (int k = 0; k < K; k++) r64[d][k] = fma(vb[dd][d], xv[k], r64[d][k]);
+          if (need_cold && nb_bc)
+#pragma unroll
+            for (int d = 0; d < 3; d++)
+#pragma unroll
+              for (int k = 0; k < K; k++) rc64[d][k] = fma(vb[dd][d], xv[k], rc64[d][k]);
         }
       }
+    }
+    TS *dv = (TS *)P.d, *rv = (TS *)P.r, *zv = (TS *)P.z;
+    const TS *dinv = (const TS *)P.dinv;
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+      const long long grow = rbase + 3LL * n + d;
       TS ri[KS], zi[KS], zero[KS];
       const TS dg = dinv[grow];
 #pragma unroll
       for (int k = 0; k < KS; k++) {
         const int kk = k < K ? k : 0;
-        const double rk = (k < K) ? -r64[kk] : 0.0;
+        const double rk = (k < K) ? -r64[d][kk] : 0.0;
         ri[k] = (TS)rk;
         zi[k] = dg * ri[k];
         zero[k] = (TS)0;
@@ -801,7 +946,7 @@ __device__ void res_slab(const EngPtrs &P, int slab, int slot, const RveDev *R, 
           v[kk] += rk * rk;
           v[K + kk] += (double)ri[k] * (double)ri[k];
           v[2 * K + kk] += (double)ri[k] * (double)zi[k];
-          v[3 * K + kk] += rc64[kk] * rc64[kk];
+          v[3 * K + kk] += rc64[d][kk] * rc64[d][kk];
         }
       }
       store_vec<TS, KS>(rv + grow * KS, ri);
@@ -893,11 +1038,11 @@ __global__ __launch_bounds__(256) void k_step_r(EngPtrs P, SolveParams sp)
 
 // Stand-alone copy of the INNER-phase SpMV (w = A z) over every resident RVE,
 // written to q; used only when no RVE is being solved.
-template <typename TS, int K>
+template <typename TS, int K, bool DICT>
 __global__ __launch_bounds__(256) void k_spmv_time(EngPtrs P)
 {
   constexpr int KS = vec_stride(K);
-  const int slab = xcd_slab(), tid = threadIdx.x, SN = P.SN;
+  const int slab = xcd_slab(), tid = threadIdx.x;
   const int slot = P.slab_rve[slab];
   if (slot < 0) return;
   const RveDev *R = P.rves + slot;
@@ -905,8 +1050,7 @@ __global__ __launch_bounds__(256) void k_spmv_time(EngPtrs P)
   if (n >= R->nrow / 3) return;
   const long long rbase = R->row_base;
   TS w[3][KS];
-  node_spmv<TS, KS>(P.bcols, (const TS *)P.vals, (long long)slab * 27 * SN, (long long)slab * kW * 3 * SN,
-                    SN, tid, rbase, (const TS *)P.z, w);
+  slab_node_spmv<TS, KS, DICT>(P, R, slot, slab, tid, n, w);
   TS o[3 * KS];
 #pragma unroll
   for (int d = 0; d < 3; d++)
@@ -918,12 +1062,12 @@ __global__ __launch_bounds__(256) void k_spmv_time(EngPtrs P)
 // ---------------------------------------------------------------------------
 // launchers (explicit instantiations for the supported load-case counts)
 
-template <typename TS, int K>
+template <typename TS, int K, bool DICT>
 static int launch_steps(const EngPtrs &P, const SolveParams &sp, int grid, int nsteps, hipStream_t s)
 {
   if (grid % kXcd) return MM_ERR_ARG;  // the XCD-aware slab order needs whole groups of 8
   for (int i = 0; i < nsteps; i++) {
-    k_step_a<TS, K><<<grid, P.SN, 3 * P.SN * vec_stride(K) * sizeof(TS), s>>>(P, sp);
+    k_step_a<TS, K, DICT><<<grid, P.SN, 3 * P.SN * vec_stride(K) * sizeof(TS), s>>>(P, sp);
     k_step_b<TS, K><<<grid, P.SN, 0, s>>>(P, sp);
     k_step_r<TS, K><<<grid, P.SN, 0, s>>>(P, sp);
   }
@@ -937,15 +1081,16 @@ static int launch_assemble(const EngPtrs &P, int nblk, const int *slab_list, int
   if (nblk <= 0) return 0;
   k_assemble<TS, K><<<nblk, P.SN, 0, s>>>(nblk, slab_list, P.slab_rve, P.slab_node0, P.rves, P.ke_tab,
                                           P.elem_mat, const_cast<int *>(P.bcols), (TS *)P.vals,
-                                          (TS *)P.dinv, P.x64, P.SN, mode);
+                                          (TS *)P.dinv, const_cast<uint8_t *>(P.npat), P.x64, P.SN,
+                                          mode);
   MM_HIP(hipGetLastError());
   return 0;
 }
 
-template <typename TS, int K>
+template <typename TS, int K, bool DICT>
 static int launch_time(const EngPtrs &P, int grid, hipStream_t s)
 {
-  k_spmv_time<TS, K><<<grid, P.SN, 0, s>>>(P);
+  k_spmv_time<TS, K, DICT><<<grid, P.SN, 0, s>>>(P);
   MM_HIP(hipGetLastError());
   return 0;
 }
@@ -971,12 +1116,26 @@ static int launch_time(const EngPtrs &P, int grid, hipStream_t s)
 
 bool eng_supported_rhs(int K) { return K == 1 || K == 6 || K == 7; }
 
-int eng_launch_steps(bool fp64, int K, const EngPtrs &P, const SolveParams &sp, int grid, int nsteps,
-                     hipStream_t s)
+int eng_launch_steps(bool fp64, int K, bool dict, const EngPtrs &P, const SolveParams &sp, int grid,
+                     int nsteps, hipStream_t s)
 {
-#define CALL_STEPS(T, KK) launch_steps<T, KK>(P, sp, grid, nsteps, s)
+  if (dict) {
+#define CALL_STEPS(T, KK) launch_steps<T, KK, true>(P, sp, grid, nsteps, s)
+    MM_DISPATCH(fp64, K, CALL_STEPS);
+#undef CALL_STEPS
+  }
+#define CALL_STEPS(T, KK) launch_steps<T, KK, false>(P, sp, grid, nsteps, s)
   MM_DISPATCH(fp64, K, CALL_STEPS);
 #undef CALL_STEPS
+}
+
+int eng_launch_dict(bool fp64, int n, const int *slots, const double *ke_tab, void *dict, hipStream_t s)
+{
+  if (n <= 0) return 0;
+  if (fp64) k_dict<double><<<n * kDictPat, 384, 0, s>>>(n, slots, ke_tab, (double *)dict);
+  else k_dict<float><<<n * kDictPat, 384, 0, s>>>(n, slots, ke_tab, (float *)dict);
+  MM_HIP(hipGetLastError());
+  return 0;
 }
 
 int eng_launch_assemble(bool fp64, int K, const EngPtrs &P, int nblk, const int *slab_list, int mode,
@@ -987,9 +1146,14 @@ int eng_launch_assemble(bool fp64, int K, const EngPtrs &P, int nblk, const int 
 #undef CALL_ASM
 }
 
-int eng_launch_time(bool fp64, int K, const EngPtrs &P, int grid, hipStream_t s)
+int eng_launch_time(bool fp64, int K, bool dict, const EngPtrs &P, int grid, hipStream_t s)
 {
-#define CALL_TIME(T, KK) launch_time<T, KK>(P, grid, s)
+  if (dict) {
+#define CALL_TIME(T, KK) launch_time<T, KK, true>(P, grid, s)
+    MM_DISPATCH(fp64, K, CALL_TIME);
+#undef CALL_TIME
+  }
+#define CALL_TIME(T, KK) launch_time<T, KK, false>(P, grid, s)
   MM_DISPATCH(fp64, K, CALL_TIME);
 #undef CALL_TIME
 }

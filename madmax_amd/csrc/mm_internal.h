@@ -143,17 +143,31 @@ struct EngPtrs {
   RveOut *out;
   const double *ke_tab, *c_tab, *bw_tab;
   const uint8_t *elem_mat;
+  const uint8_t *npat;  // [slab][SN] material pattern of the node's 8 elements
+  const void *dict;     // [rve slot][256][27][12] interior 3-row ELL blocks (TS)
+  const double *dict64; // the same in FP64 (the residual's matrix)
   int SN;  // nodes per slab = lanes per workgroup
 };
+
+// Dictionary ELL: entry (slot, pattern) holds, for the 27 neighbour blocks t
+// of an interior node in the reference's slot order, v[d'][d] (value of row
+// 3n+d, column 3m+d') at [t * 12 + d' * 4 + d] (padded to 16-byte rows).
+constexpr int kDictPat = 256;
+constexpr int kDictBlk = 12;
+constexpr int kDictEntry = 27 * kDictBlk;
+
+// assembly modes
+enum AsmMode : int { ASM_FULL = 0, ASM_X_ONLY = 1, ASM_DICT = 2 };
 
 // launchers (engine_kernels.hip)
 int upload_slot_tables(hipStream_t s);
 bool eng_supported_rhs(int K);
-int eng_launch_steps(bool fp64, int K, const EngPtrs &P, const SolveParams &sp, int grid, int nsteps,
-                     hipStream_t s);
+int eng_launch_steps(bool fp64, int K, bool dict, const EngPtrs &P, const SolveParams &sp, int grid,
+                     int nsteps, hipStream_t s);
+int eng_launch_dict(bool fp64, int n, const int *slots, const double *ke_tab, void *dict, hipStream_t s);
 int eng_launch_assemble(bool fp64, int K, const EngPtrs &P, int nblk, const int *slab_list, int mode,
                         hipStream_t s);
-int eng_launch_time(bool fp64, int K, const EngPtrs &P, int grid, hipStream_t s);
+int eng_launch_time(bool fp64, int K, bool dict, const EngPtrs &P, int grid, hipStream_t s);
 int eng_launch_admit(int n, const int *slots, const RveDev *states, RveDev *rves, int *slab_rve,
                      int *slab_node0, int SN, hipStream_t s);
 int eng_launch_ke(int n, const int *work, const RveDev *rves, double *ke_tab, double *c_tab_out,

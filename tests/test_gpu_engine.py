@@ -9,8 +9,8 @@ import time
 import numpy as np
 import pytest
 
-from madmax_amd import (MM_PREC_FP64, MM_PREC_MIXED, MM_TANGENT_FD, MM_TANGENT_UNIT, Engine,
-                        configs, fd_loads, rve_spec, unit_loads)
+from madmax_amd import (MM_ELL_DICT, MM_ELL_FULL, MM_PREC_FP64, MM_PREC_MIXED, MM_TANGENT_FD,
+                        MM_TANGENT_UNIT, Engine, configs, fd_loads, rve_spec, unit_loads)
 from madmax_amd import _lib
 from tests.helpers import normwise_rel, oracle_rve
 
@@ -36,11 +36,13 @@ def _solve_batch(eng, specs, key0=0):
     return [res[i + 1] for i in range(len(specs))]
 
 
-def test_fp32_slabs_equal_rounded_oracle():
+@pytest.mark.parametrize("storage", [MM_ELL_FULL, MM_ELL_DICT])
+def test_fp32_slabs_equal_rounded_oracle(storage):
     spec = rve_spec(5, [(0.4, 0.6, 0.3)])
     ref, et = oracle_rve(spec)
     cols_ref, vals_ref = ref.assemble()
-    with Engine(nrhs=6, precision=MM_PREC_MIXED, hbm_budget=GB // 4, max_rves=8) as eng:
+    with Engine(nrhs=6, precision=MM_PREC_MIXED, hbm_budget=GB // 4, max_rves=8,
+                ell_storage=storage) as eng:
         _solve_batch(eng, [spec], key0=100)
         cols, vals = eng.download_ell(101)
         assert np.array_equal(eng.download_phase(101), et)
@@ -136,7 +138,7 @@ def test_serving_thread_admits_while_busy():
 
 def test_admission_limits():
     with Engine(nrhs=6, hbm_budget=64 << 20, max_rves=4) as eng:
-        big = rve_spec(40)
+        big = rve_spec(64)  # ~0.8 M rows of vectors alone exceed 64 MB
         assert eng.submit(1, big, unit_loads(), MM_TANGENT_UNIT) == _lib.MM_REJECTED
         bad = rve_spec(4, young=(float("nan"), 1e8))
         assert eng.submit(2, bad, unit_loads(), MM_TANGENT_UNIT) == _lib.MM_ERR_ARG
@@ -183,3 +185,32 @@ def test_refinement_sweep_targets():
         assert normwise_rel(b["c_tangent"], c_ref) <= TOL
         assert a["sweeps"] > b["sweeps"]
         assert a["residual_rel"] <= 1e-10 and b["residual_rel"] <= 1e-10
+
+
+@pytest.mark.parametrize("precision", [MM_PREC_FP64, MM_PREC_MIXED])
+def test_dictionary_storage_equals_full_ell(precision):
+    """The dictionary rows are the assembled rows bit for bit and the product
+    keeps the slot order, so the whole solve is the same to the last bit."""
+    specs = [configs.c4_spec(11, i) for i in range(8)]
+    specs = [s for s in specs if s.nex <= 20][:3] + [rve_spec((5, 4, 6), [(0.3, 0.6, 0.25)])]
+    out = {}
+    for storage in (MM_ELL_FULL, MM_ELL_DICT):
+        with Engine(nrhs=6, precision=precision, hbm_budget=GB, max_rves=16, ell_storage=storage,
+                    inner_rtol=1e-10 if precision == MM_PREC_FP64 else 1e-5) as eng:
+            out[storage] = _solve_batch(eng, specs, key0=40)
+            cols, vals = eng.download_ell(40 + len(specs))
+            out[storage, "ell"] = (cols, vals)
+    for a, b in zip(out[MM_ELL_FULL], out[MM_ELL_DICT]):
+        assert a["status"] == 0 and b["status"] == 0
+        assert np.array_equal(a["c_tangent"], b["c_tangent"])
+        assert a["pcg_its"] == b["pcg_its"] and a["sweeps"] == b["sweeps"]
+    assert np.array_equal(out[MM_ELL_FULL, "ell"][0], out[MM_ELL_DICT, "ell"][0])
+    assert np.array_equal(out[MM_ELL_FULL, "ell"][1], out[MM_ELL_DICT, "ell"][1])
+
+
+@pytest.mark.parametrize("storage", [MM_ELL_FULL, MM_ELL_DICT])
+def test_spmv_timing_runs(storage):
+    with Engine(nrhs=6, hbm_budget=GB, max_rves=8, ell_storage=storage) as eng:
+        _solve_batch(eng, [configs.c2_spec(3, i, n=12) for i in range(2)], key0=60)
+        ms, nbytes = eng.time_spmv(reps=3)
+    assert ms > 0 and nbytes > 0
