@@ -149,7 +149,7 @@ int ensure_device(T **p, size_t *cap, size_t n)
 
 struct mm_engine {
   mm_engine_config cfg;
-  int SN = 128, K = 1, KS = 1;
+  int SN = 128, K = 1;
   bool fp64 = false;
   bool dict_mode = false;  // MM_ELL_DICT: no per-row ELL arrays, dictionary + pattern bytes
   size_t tsz = 4;
@@ -230,6 +230,7 @@ struct mm_engine {
     P.dict = dict;
     P.dict64 = dict64;
     P.SN = SN;
+    P.sn_shift = __builtin_ctz((unsigned)SN);
     return P;
   }
   SolveParams params() const
@@ -253,7 +254,7 @@ struct mm_engine {
   {
     const size_t RS = (size_t)rows_per_slab();
     const size_t ell = dict_mode ? 0 : RS * kW * tsz + (size_t)SN * 27 * 4;
-    return ell + RS * (KS * (8 + 5 * tsz) + tsz) + kPartDoubles * 8 + 8 + (size_t)SN * 2 + 1;
+    return ell + RS * (K * (8 + 5 * tsz) + tsz) + kPartDoubles * 8 + 8 + (size_t)SN * 2 + 1;
   }
 };
 
@@ -323,7 +324,6 @@ extern "C" int mm_engine_create(const mm_engine_config *cfg, mm_engine **out)
   }
   e->SN = cfg->slab_nodes ? cfg->slab_nodes : 128;
   e->K = cfg->nrhs;
-  e->KS = vec_stride_host(e->K);
   e->fp64 = (cfg->precision == MM_PREC_FP64);
   e->tsz = e->fp64 ? 8 : 4;
   e->dict_mode = (cfg->ell_storage == MM_ELL_DICT);
@@ -380,12 +380,12 @@ extern "C" int mm_engine_create(const mm_engine_config *cfg, mm_engine **out)
   }
   EALLOC(e->npat, (size_t)e->nslab_cap * e->SN);
   EALLOC(e->dinv, nrows * e->tsz);
-  EALLOC(e->x64, nrows * e->KS * 8);
-  EALLOC(e->d, nrows * e->KS * e->tsz);
-  EALLOC(e->r, nrows * e->KS * e->tsz);
-  EALLOC(e->z, nrows * e->KS * e->tsz);
-  EALLOC(e->p, nrows * e->KS * e->tsz);
-  EALLOC(e->q, nrows * e->KS * e->tsz);
+  EALLOC(e->x64, nrows * e->K * 8);
+  EALLOC(e->d, nrows * e->K * e->tsz);
+  EALLOC(e->r, nrows * e->K * e->tsz);
+  EALLOC(e->z, nrows * e->K * e->tsz);
+  EALLOC(e->p, nrows * e->K * e->tsz);
+  EALLOC(e->q, nrows * e->K * e->tsz);
   EALLOC(e->part, (size_t)e->nslab_cap * kPartDoubles * 8);
   EALLOC(e->slab_rve, (size_t)e->nslab_cap * 4);
   EALLOC(e->slab_node0, (size_t)e->nslab_cap * 4);
@@ -1002,11 +1002,16 @@ extern "C" int mm_engine_download_x(mm_engine *e, uint64_t key, int load, double
   const int s = find_cached(e, key);
   if (s < 0) return MM_ERR_STATE;
   const HostSlot &h = e->slots[s];
-  std::vector<double> hx((size_t)h.nrow * e->KS);
-  MM_HIP(hipMemcpyAsync(hx.data(), e->x64 + (size_t)h.first_slab * e->rows_per_slab() * e->KS,
-                        hx.size() * sizeof(double), hipMemcpyDeviceToHost, e->stream));
+  const size_t RS = (size_t)e->rows_per_slab();
+  std::vector<double> hx((size_t)h.nslab * RS * e->K);
+  MM_HIP(hipMemcpyAsync(hx.data(), e->x64 + (size_t)h.first_slab * RS * e->K, hx.size() * sizeof(double),
+                        hipMemcpyDeviceToHost, e->stream));
   MM_HIP(hipStreamSynchronize(e->stream));
-  for (int i = 0; i < h.nrow; i++) x[i] = hx[(size_t)i * e->KS + load];
+  // [slab][K][3*SN] -> row-major for one load case
+  for (int i = 0; i < h.nrow; i++) {
+    const size_t sl = (size_t)i / RS, lr = (size_t)i % RS;
+    x[i] = hx[(sl * e->K + load) * RS + lr];
+  }
   return 0;
 }
 

@@ -10,8 +10,8 @@
 //   bcols : [slab][27][SN]    RVE-local node id of slot block t (-1 = empty);
 //                             the reference's column of slot 3t + d' is
 //                             3*bcols + d', shared by the node's three rows
-//   vectors [row][KS]         x64 (FP64 displacement), d (correction), r, z,
-//                             p, q (PCG), KS = load cases padded to even
+//   vectors [slab][K][3*SN]   x64 (FP64 displacement), d (correction), r, z,
+//                             p, q (PCG); K = load cases
 //   dinv  : [row]             Jacobi preconditioner
 //   part  : [slab][96]        per-workgroup partial sums (FP64)
 // Every resident RVE owns a contiguous run of slabs; its first row is
@@ -43,7 +43,6 @@ int upload_slot_tables(hipStream_t s)
 
 constexpr int kPart = kPartDoubles;
 
-__host__ __device__ constexpr int vec_stride(int K) { return (K == 1) ? 1 : ((K + 1) & ~1); }
 
 // MI355X dispatches workgroup i to XCD i % 8.  The solver grids are a
 // multiple of 8 and workgroup i processes slab (i % 8) * (grid / 8) + i / 8,
@@ -210,7 +209,6 @@ __global__ void k_assemble(int nblk, const int *__restrict__ slab_list, const in
                            uint8_t *__restrict__ npat, double *__restrict__ x64, int SN, int mode)
 {
 #pragma clang fp contract(off)
-  constexpr int KS = vec_stride(K);
   if ((int)blockIdx.x >= nblk) return;
   const int slab = slab_list[blockIdx.x];
   const int slot = slab_rve[slab];
@@ -221,6 +219,7 @@ __global__ void k_assemble(int nblk, const int *__restrict__ slab_list, const in
   const long long vb = (long long)slab * kW * RS;
   const long long cb = (long long)slab * 27 * SN;
   const long long grow0 = R.row_base + 3LL * n;
+  const long long xb = (long long)slab * K * RS + 3 * tid;  // node n in the [slab][K][3*SN] vectors
   const int nn = R.nrow / 3;
   if (n >= nn) {
     if (mode == ASM_FULL) {
@@ -233,7 +232,7 @@ __global__ void k_assemble(int nblk, const int *__restrict__ slab_list, const in
       for (int d = 0; d < 3; d++) dinv[grow0 + d] = (TV)0;
     }
     for (int d = 0; d < 3; d++)
-      for (int k = 0; k < KS; k++) x64[(grow0 + d) * KS + k] = 0.0;
+      for (int k = 0; k < K; k++) x64[xb + (long long)k * RS + d] = 0.0;
     return;
   }
   const int nx = R.nx, nxy = R.nx * R.ny;
@@ -312,16 +311,16 @@ __global__ void k_assemble(int nblk, const int *__restrict__ slab_list, const in
   // initial displacement for every load case
   const double X[3] = {ix * R.hx, iy * R.hy, iz * R.hz};
   for (int d = 0; d < 3; d++)
-    for (int k = 0; k < KS; k++) {
+    for (int k = 0; k < K; k++) {
       if (k < R.nload && bc) {
         const double *s = R.load[k];
         double u;
         if (d == 0) u = s[0] * X[0] + s[3] / 2 * X[1] + s[5] / 2 * X[2];
         else if (d == 1) u = s[3] / 2 * X[0] + s[1] * X[1] + s[4] / 2 * X[2];
         else u = s[5] / 2 * X[0] + s[4] / 2 * X[1] + s[2] * X[2];
-        x64[(grow0 + d) * KS + k] = u;
+        x64[xb + (long long)k * RS + d] = u;
       } else if (!(R.warm && k < R.nload)) {
-        x64[(grow0 + d) * KS + k] = 0.0;
+        x64[xb + (long long)k * RS + d] = 0.0;
       }
     }
 }
@@ -397,18 +396,52 @@ __device__ __forceinline__ void store_vec(TS *__restrict__ p, const TS (&v)[N])
   }
 }
 
-// w = A z for the three rows of one node and KS load cases: 27 slot blocks,
-// each one column id, a 3x3 value block and one gathered 3*KS vector run.
-template <typename TS, int KS>
-__device__ __forceinline__ void node_spmv(const int *__restrict__ bcols, const TS *__restrict__ vals,
-                                          long long cb, long long vb, int SN, int tid, long long rbase,
-                                          const TS *__restrict__ z, TS (&w)[3][KS])
+// ---------------------------------------------------------------------------
+// Solver vectors are [slab][K][3*SN]: within a slab, each load case is one
+// run of the slab's 3*SN rows.  Consecutive lanes (nodes) then read
+// consecutive 12-byte triples of one load case, so a wavefront's gather of a
+// neighbour block is one contiguous run, and the element-wise passes stream
+// each load case's run with a compile-time case index.
+
+// Offset of RVE-local node m's first row in load case 0.
+__device__ __forceinline__ long long node_vec(const EngPtrs &P, int first_slab, int m, int K)
 {
-  const int RS = 3 * SN;
+  return ((long long)(first_slab + (m >> P.sn_shift)) * K) * (3 * P.SN) + 3 * (m & (P.SN - 1));
+}
+
+template <typename T, int K>
+__device__ __forceinline__ void gather_node(const T *__restrict__ x, long long nb, int RS, T (&v)[3][K])
+{
+#pragma unroll
+  for (int k = 0; k < K; k++)
+#pragma unroll
+    for (int dd = 0; dd < 3; dd++) v[dd][k] = x[nb + (long long)k * RS + dd];
+}
+
+template <typename T, int K>
+__device__ __forceinline__ void scatter_node(T *__restrict__ x, long long nb, int RS, const T (&v)[3][K])
+{
+#pragma unroll
+  for (int k = 0; k < K; k++)
+#pragma unroll
+    for (int d = 0; d < 3; d++) x[nb + (long long)k * RS + d] = v[d][k];
+}
+
+// Row block of lane tid (node n) times z, full storage: the 27 slot blocks in
+// the reference's slot order, w[d] accumulated over slots j = 3t + d'.
+// Load cases set in `skip` are not needed and are left out.
+template <typename TS, int K>
+__device__ __forceinline__ void node_spmv(const EngPtrs &P, int first_slab, int slab, int tid, unsigned skip,
+                                          const TS *__restrict__ z, TS (&w)[3][K])
+{
+  const int SN = P.SN, RS = 3 * SN;
+  const int *__restrict__ bcols = P.bcols;
+  const TS *__restrict__ vals = (const TS *)P.vals;
+  const long long cb = (long long)slab * 27 * SN, vb = (long long)slab * kW * RS;
 #pragma unroll
   for (int d = 0; d < 3; d++)
 #pragma unroll
-    for (int k = 0; k < KS; k++) w[d][k] = (TS)0;
+    for (int k = 0; k < K; k++) w[d][k] = (TS)0;
 #pragma unroll 3
   for (int t = 0; t < 27; t++) {
     const int m = __builtin_nontemporal_load(&bcols[cb + (long long)t * SN + tid]);
@@ -419,14 +452,18 @@ __device__ __forceinline__ void node_spmv(const int *__restrict__ bcols, const T
       for (int d = 0; d < 3; d++)
         v[dd][d] = __builtin_nontemporal_load(&vals[vb + (long long)(3 * t + dd) * RS + 3 * tid + d]);
     if (m >= 0) {
-      TS xz[3 * KS];
-      load_vec<TS, 3 * KS>(z + (rbase + 3LL * m) * KS, xz);
+      const long long nb = node_vec(P, first_slab, m, K);
 #pragma unroll
-      for (int dd = 0; dd < 3; dd++)
+      for (int k = 0; k < K; k++) {
+        if ((skip >> k) & 1u) continue;
+        TS x[3];
 #pragma unroll
-        for (int d = 0; d < 3; d++)
+        for (int dd = 0; dd < 3; dd++) x[dd] = z[nb + (long long)k * RS + dd];
 #pragma unroll
-          for (int k = 0; k < KS; k++) w[d][k] = fma(v[dd][d], xz[dd * KS + k], w[d][k]);
+        for (int dd = 0; dd < 3; dd++)
+#pragma unroll
+          for (int d = 0; d < 3; d++) w[d][k] = fma(v[dd][d], x[dd], w[d][k]);
+      }
     }
   }
 }
@@ -436,25 +473,28 @@ __device__ __forceinline__ void node_spmv(const int *__restrict__ bcols, const T
 // the values stay in L1/L2) at the structured neighbour offsets; a boundary
 // node's row is the identity, whose full-ELL product z + 0 is reproduced
 // exactly.  Same fma order as node_spmv, so the same bits.
-template <typename TS, int KS>
-__device__ __forceinline__ void node_spmv_dict(const TS *__restrict__ dict_rve, int pat, bool bc, int n,
-                                               int nx, int nxy, long long rbase,
-                                               const TS *__restrict__ z, TS (&w)[3][KS])
+template <typename TS, int K>
+__device__ __forceinline__ void node_spmv_dict(const EngPtrs &P, const RveDev *R, int slot, int slab, int tid,
+                                               int n, unsigned skip, const TS *__restrict__ z, TS (&w)[3][K])
 {
+  const int RS = 3 * P.SN;
+  const int nx = R->nx, nxy = R->nx * R->ny;
+  const int ix = n % nx, iy = (n / nx) % R->ny, iz = n / nxy;
+  const bool bc = node_class(ix, iy, iz, nx, R->ny, R->nz) != 13;
   if (bc) {
-    TS xz[3 * KS];
-    load_vec<TS, 3 * KS>(z + (rbase + 3LL * n) * KS, xz);
+    const long long nb = node_vec(P, R->first_slab, n, K);
 #pragma unroll
-    for (int d = 0; d < 3; d++)
+    for (int k = 0; k < K; k++)
 #pragma unroll
-      for (int k = 0; k < KS; k++) w[d][k] = xz[d * KS + k] + (TS)0;
+      for (int d = 0; d < 3; d++) w[d][k] = ((skip >> k) & 1u) ? (TS)0 : z[nb + (long long)k * RS + d] + (TS)0;
     return;
   }
 #pragma unroll
   for (int d = 0; d < 3; d++)
 #pragma unroll
-    for (int k = 0; k < KS; k++) w[d][k] = (TS)0;
-  const TS *D = dict_rve + (size_t)pat * kDictEntry;
+    for (int k = 0; k < K; k++) w[d][k] = (TS)0;
+  const TS *D = (const TS *)P.dict +
+                ((size_t)slot * kDictPat + P.npat[(long long)slab * P.SN + tid]) * kDictEntry;
 #pragma unroll 3
   for (int t = 0; t < 27; t++) {
     const int key = g_slots.off[13][t];
@@ -462,70 +502,42 @@ __device__ __forceinline__ void node_spmv_dict(const TS *__restrict__ dict_rve, 
     TS v[3][4];  // [d'][d]
 #pragma unroll
     for (int dd = 0; dd < 3; dd++) load_vec<TS, 4>(D + t * kDictBlk + dd * 4, v[dd]);
-    TS xz[3 * KS];
-    load_vec<TS, 3 * KS>(z + (rbase + 3LL * m) * KS, xz);
+    const long long nb = node_vec(P, R->first_slab, m, K);
 #pragma unroll
-    for (int dd = 0; dd < 3; dd++)
+    for (int k = 0; k < K; k++) {
+      if ((skip >> k) & 1u) continue;
+      TS x[3];
 #pragma unroll
-      for (int d = 0; d < 3; d++)
+      for (int dd = 0; dd < 3; dd++) x[dd] = z[nb + (long long)k * RS + dd];
 #pragma unroll
-        for (int k = 0; k < KS; k++) w[d][k] = fma(v[dd][d], xz[dd * KS + k], w[d][k]);
+      for (int dd = 0; dd < 3; dd++)
+#pragma unroll
+        for (int d = 0; d < 3; d++) w[d][k] = fma(v[dd][d], x[dd], w[d][k]);
+    }
   }
 }
 
-// Product w = A z for the node of lane tid of a slab, from either storage.
-template <typename TS, int KS, bool DICT>
-__device__ __forceinline__ void slab_node_spmv(const EngPtrs &P, const RveDev *R, int slot, int slab,
-                                               int tid, int n, TS (&w)[3][KS])
+template <typename TS, int K, bool DICT>
+__device__ __forceinline__ void slab_node_spmv(const EngPtrs &P, const RveDev *R, int slot, int slab, int tid,
+                                               int n, unsigned skip, TS (&w)[3][K])
 {
-  const int SN = P.SN;
-  if (DICT) {
-    const int nx = R->nx, nxy = R->nx * R->ny;
-    const int ix = n % nx, iy = (n / nx) % R->ny, iz = n / nxy;
-    const bool bc = node_class(ix, iy, iz, nx, R->ny, R->nz) != 13;
-    const TS *dict_rve = (const TS *)P.dict + (size_t)slot * kDictPat * kDictEntry;
-    node_spmv_dict<TS, KS>(dict_rve, P.npat[(long long)slab * SN + tid], bc, n, nx, nxy, R->row_base,
-                           (const TS *)P.z, w);
-  } else {
-    node_spmv<TS, KS>(P.bcols, (const TS *)P.vals, (long long)slab * 27 * SN,
-                      (long long)slab * kW * 3 * SN, SN, tid, R->row_base, (const TS *)P.z, w);
-  }
+  if (DICT) node_spmv_dict<TS, K>(P, R, slot, slab, tid, n, skip, (const TS *)P.z, w);
+  else node_spmv<TS, K>(P, R->first_slab, slab, tid, skip, (const TS *)P.z, w);
 }
 
-// ---------------------------------------------------------------------------
-// Element-wise passes over a slab's vectors.  A slab's rows are contiguous in
-// every [row][KS] vector, so a lane handles VEC consecutive values (VEC = 2
-// for even KS) and a wavefront's access is one contiguous run.  The lanes
-// stride by S, the largest count <= blockDim with S * VEC a multiple of KS,
-// so lane tid always meets the same load cases k0 = (tid * VEC) % KS ..
-// k0 + VEC - 1: their scalars stay in registers and the dot products
-// accumulate in VEC plain scalars (a per-value case index would put the
-// accumulators in scratch memory).
-template <int KS>
-struct SlabVec {
-  static constexpr int VEC = (KS % 2 == 0) ? 2 : 1;
+// Element-wise passes: lane tid handles the value pairs tid, tid + blockDim,
+// ... of each load case's run of `rows` valid rows (a pair is 8 contiguous
+// bytes in FP32), the odd last row by lane 0.
+template <typename TS>
+struct Pair;
+template <>
+struct Pair<float> {
+  using type = float2;
 };
-
-template <int KS, int VEC>
-__device__ __forceinline__ int fixed_case_stride()
-{
-  int S = blockDim.x;
-  while ((S * VEC) % KS) S--;
-  return S;
-}
-
-// v[k] = the lane's accumulator of load case k (zero for cases it never met).
-template <int K, int VEC>
-__device__ __forceinline__ void route_cases(const double (&a)[VEC], int k0, double (&v)[K])
-{
-#pragma unroll
-  for (int k = 0; k < K; k++) {
-    double s = 0.0;
-#pragma unroll
-    for (int j = 0; j < VEC; j++) s = (k == k0 + j) ? a[j] : s;
-    v[k] = s;
-  }
-}
+template <>
+struct Pair<double> {
+  using type = double2;
+};
 
 // Per-slab partial sums of the launch that produced them, summed over the
 // RVE's slabs in slab order (group_sum): every workgroup of the consuming
@@ -667,22 +679,18 @@ __device__ void finish_rve(const EngPtrs &P, int slot, const RveDev *R, SolveSta
 // Kernel A (reads st[2], writes st[0]).
 // PH_RESRED: starts the sweep from the residual norms, then continues as
 // INNER with beta = 0.  PH_INNER: from the last B's r.r and r.z, either ends
-// the sweep (x64 += d, PH_RES) or forms beta and does w = A z (one lane per
-// node, staged in LDS), p = z + beta p, q = w + beta q (so q = A p without
-// gathering p) and the partial p.q.  PH_AVGRED: the first slab's workgroup
+// the sweep (x64 += d, PH_RES) or forms beta and, one lane per node, does
+// w = A z, then on the node's own rows p = z + beta p, q = w + beta q (so
+// q = A p without gathering p) and the partial p.q.  Load cases that have
+// met their sweep target are skipped.  PH_AVGRED: the first slab's workgroup
 // writes the result.
 template <typename TS, int K, bool DICT>
 __global__ __launch_bounds__(256) void k_step_a(EngPtrs P, SolveParams sp)
 {
-  constexpr int KS = vec_stride(K);
-  constexpr int VEC = SlabVec<KS>::VEC;
-  extern __shared__ __align__(16) unsigned char smem_a[];
-  TS *ws = reinterpret_cast<TS *>(smem_a);  // [3*SN][KS]
   __shared__ double lds[4 * K * 16];
   __shared__ SolveState N;
-  __shared__ TS s_beta[KS];
   __shared__ int s_xupd;
-  const int slab = xcd_slab(), tid = threadIdx.x, SN = P.SN;
+  const int slab = xcd_slab(), tid = threadIdx.x, SN = P.SN, RS = 3 * SN;
   const int slot = P.slab_rve[slab];
   if (slot < 0) return;
   RveDev *R = P.rves + slot;
@@ -713,64 +721,52 @@ __global__ __launch_bounds__(256) void k_step_a(EngPtrs P, SolveParams sp)
   }
   __syncthreads();
   if (writer && tid == 0) R->st[0] = N;
-  const int phase = N.phase;
-  if (phase != PH_INNER && !s_xupd) return;
+  if (N.phase != PH_INNER && !s_xupd) return;
 
   const int n0 = P.slab_node0[slab];
-  const int rows = min(3 * SN, R->nrow - 3 * n0);  // valid rows of this slab
-  const long long rbase = R->row_base;
-  const long long f0 = (rbase + 3LL * n0) * KS;   // first value of the slab
-  const int ngroups = rows * KS / VEC;
+  const int rows = min(RS, R->nrow - 3 * n0);  // valid rows of this slab
+  const long long sb = (long long)slab * K * RS;
 
   if (s_xupd) {
     const TS *d = (const TS *)P.d;
-    for (int g = tid; g < ngroups; g += blockDim.x)
 #pragma unroll
-      for (int j = 0; j < VEC; j++) P.x64[f0 + (long long)g * VEC + j] += (double)d[f0 + (long long)g * VEC + j];
+    for (int k = 0; k < K; k++) {
+      const long long b = sb + (long long)k * RS;
+      for (int e = tid; e < rows; e += blockDim.x) P.x64[b + e] += (double)d[b + e];
+    }
     return;
   }
 
   // PH_INNER
-  if (tid < KS) s_beta[tid] = (tid < K) ? (TS)N.beta[tid] : (TS)0;
-  const int S = fixed_case_stride<KS, VEC>();
-  const int k0 = (tid * VEC) % KS;
+  const unsigned skip = N.done_in;
   const int n = n0 + tid;
-  if (n < R->nrow / 3) {
-    TS w[3][KS];
-    slab_node_spmv<TS, KS, DICT>(P, R, slot, slab, tid, n, w);
-#pragma unroll
-    for (int d = 0; d < 3; d++)
-#pragma unroll
-      for (int k = 0; k < KS; k++) ws[(3 * tid + d) * KS + k] = w[d][k];
-  }
-  __syncthreads();
-  TS *p = (TS *)P.p;
-  TS *q = (TS *)P.q;
-  const TS *z = (const TS *)P.z;
-  TS bj[VEC];
-  double acc[VEC];
-#pragma unroll
-  for (int j = 0; j < VEC; j++) {
-    bj[j] = s_beta[k0 + j];
-    acc[j] = 0.0;
-  }
-  for (int g = tid; tid < S && g < ngroups; g += S) {
-    const long long i0 = f0 + (long long)g * VEC;
-    TS zi[VEC], pi[VEC], qi[VEC];
-    load_vec<TS, VEC>(z + i0, zi);
-    load_vec<TS, VEC>(p + i0, pi);
-    load_vec<TS, VEC>(q + i0, qi);
-#pragma unroll
-    for (int j = 0; j < VEC; j++) {
-      pi[j] = fma(bj[j], pi[j], zi[j]);
-      qi[j] = fma(bj[j], qi[j], ws[g * VEC + j]);
-      acc[j] += (double)pi[j] * (double)qi[j];
-    }
-    store_vec<TS, VEC>(p + i0, pi);
-    store_vec<TS, VEC>(q + i0, qi);
-  }
   double v[K];
-  route_cases<K, VEC>(acc, k0, v);
+#pragma unroll
+  for (int k = 0; k < K; k++) v[k] = 0.0;
+  if (n < R->nrow / 3) {
+    // every load case is gathered (a per-case branch between the gathers
+    // would serialize their loads); finished cases skip only the update
+    TS w[3][K];
+    slab_node_spmv<TS, K, DICT>(P, R, slot, slab, tid, n, 0u, w);
+    TS *p = (TS *)P.p;
+    TS *q = (TS *)P.q;
+    const TS *z = (const TS *)P.z;
+    const long long nb = sb + 3 * tid;
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+      if ((skip >> k) & 1u) continue;
+      const TS b = (TS)N.beta[k];
+      const long long i = nb + (long long)k * RS;
+#pragma unroll
+      for (int d = 0; d < 3; d++) {
+        const TS pi = fma(b, p[i + d], z[i + d]);
+        const TS qi = fma(b, q[i + d], w[d][k]);
+        p[i + d] = pi;
+        q[i + d] = qi;
+        v[k] += (double)pi * (double)qi;
+      }
+    }
+  }
   block_sum<K>(v, lds);
   if (tid == 0) {
     double *part = slab_part(P, slab);
@@ -781,77 +777,77 @@ __global__ __launch_bounds__(256) void k_step_a(EngPtrs P, SolveParams sp)
 // ---------------------------------------------------------------------------
 // Kernel B (reads st[0], writes st[1]).  PH_INNER: alpha = r.z / p.q from
 // the last A's partials, then over the slab's values d += alpha p,
-// r -= alpha q, z = dinv r and the partial r.r, r.z.
+// r -= alpha q, z = dinv r and the partial r.r, r.z, for the load cases that
+// are still iterating.
 template <typename TS, int K>
 __global__ __launch_bounds__(256) void k_step_b(EngPtrs P, SolveParams sp)
 {
-  constexpr int KS = vec_stride(K);
-  constexpr int VEC = SlabVec<KS>::VEC;
+  using T2 = typename Pair<TS>::type;
   __shared__ double lds[2 * K * 16];
-  __shared__ TS s_alpha[KS];
-  const int slab = xcd_slab(), tid = threadIdx.x;
+  __shared__ double s_alpha[K];
+  const int slab = xcd_slab(), tid = threadIdx.x, RS = 3 * P.SN;
   const int slot = P.slab_rve[slab];
   if (slot < 0) return;
   RveDev *R = P.rves + slot;
   const SolveState *S = &R->st[0];
   if (slab == R->first_slab && tid == 0) R->st[1] = *S;
   if (S->phase != PH_INNER) return;
+  const unsigned skip = S->done_in;
   const int nslab = R->nslab;
   double s1[K];
   group_sum<K>(slab_part(P, R->first_slab), nslab, kPart, 0, s1, lds);
-  if (tid < KS) s_alpha[tid] = (TS)0;
-  __syncthreads();
   if (tid == 0)
-    for (int k = 0; k < K; k++) {
-      const bool done = (S->done_in >> k) & 1u;
-      s_alpha[k] = (TS)((done || s1[k] == 0.0) ? 0.0 : S->rz[k] / s1[k]);
-    }
+    for (int k = 0; k < K; k++)
+      s_alpha[k] = (((skip >> k) & 1u) || s1[k] == 0.0) ? 0.0 : S->rz[k] / s1[k];
   __syncthreads();
   const int n0 = P.slab_node0[slab];
-  const int rows = min(3 * P.SN, R->nrow - 3 * n0);
-  const long long row0 = R->row_base + 3LL * n0;
-  const long long f0 = row0 * KS;
-  const int ngroups = rows * KS / VEC;
+  const int rows = min(RS, R->nrow - 3 * n0);
+  const int npair = rows >> 1;
+  const long long sb = (long long)slab * K * RS;
+  const TS *dinv = (const TS *)P.dinv + (long long)slab * RS;
   TS *dv = (TS *)P.d, *rv = (TS *)P.r, *zv = (TS *)P.z;
-  const TS *p = (const TS *)P.p, *q = (const TS *)P.q, *dinv = (const TS *)P.dinv;
-  const int LS = fixed_case_stride<KS, VEC>();
-  const int k0 = (tid * VEC) % KS;
-  TS aj[VEC];
-  double arr[VEC], arz[VEC];
-#pragma unroll
-  for (int j = 0; j < VEC; j++) {
-    aj[j] = s_alpha[k0 + j];
-    arr[j] = 0.0;
-    arz[j] = 0.0;
-  }
-  for (int g = tid; tid < LS && g < ngroups; g += LS) {
-    const long long i0 = f0 + (long long)g * VEC;
-    const TS dg = dinv[row0 + (g * VEC) / KS];
-    TS di[VEC], ri[VEC], pi[VEC], qi[VEC], zi[VEC];
-    load_vec<TS, VEC>(dv + i0, di);
-    load_vec<TS, VEC>(rv + i0, ri);
-    load_vec<TS, VEC>(p + i0, pi);
-    load_vec<TS, VEC>(q + i0, qi);
-#pragma unroll
-    for (int j = 0; j < VEC; j++) {
-      di[j] = fma(aj[j], pi[j], di[j]);
-      ri[j] = fma(-aj[j], qi[j], ri[j]);
-      zi[j] = dg * ri[j];
-      arr[j] += (double)ri[j] * (double)ri[j];
-      arz[j] += (double)ri[j] * (double)zi[j];
-    }
-    store_vec<TS, VEC>(dv + i0, di);
-    store_vec<TS, VEC>(rv + i0, ri);
-    store_vec<TS, VEC>(zv + i0, zi);
-  }
-  double v[K], u[K];
-  route_cases<K, VEC>(arr, k0, v);
-  route_cases<K, VEC>(arz, k0, u);
+  const TS *p = (const TS *)P.p, *q = (const TS *)P.q;
   double vu[2 * K];
 #pragma unroll
   for (int k = 0; k < K; k++) {
-    vu[k] = v[k];
-    vu[K + k] = u[k];
+    vu[k] = 0.0;
+    vu[K + k] = 0.0;
+    if ((skip >> k) & 1u) continue;
+    const TS a = (TS)s_alpha[k];
+    const long long kb = sb + (long long)k * RS;
+    for (int g = tid; g < npair; g += blockDim.x) {
+      const long long i = kb + 2 * g;
+      const T2 dg = *reinterpret_cast<const T2 *>(dinv + 2 * g);
+      T2 di = *reinterpret_cast<const T2 *>(dv + i);
+      T2 ri = *reinterpret_cast<const T2 *>(rv + i);
+      const T2 pi = *reinterpret_cast<const T2 *>(p + i);
+      const T2 qi = *reinterpret_cast<const T2 *>(q + i);
+      di.x = fma(a, pi.x, di.x);
+      di.y = fma(a, pi.y, di.y);
+      ri.x = fma(-a, qi.x, ri.x);
+      ri.y = fma(-a, qi.y, ri.y);
+      T2 zi;
+      zi.x = dg.x * ri.x;
+      zi.y = dg.y * ri.y;
+      *reinterpret_cast<T2 *>(dv + i) = di;
+      *reinterpret_cast<T2 *>(rv + i) = ri;
+      *reinterpret_cast<T2 *>(zv + i) = zi;
+      vu[k] += (double)ri.x * (double)ri.x;
+      vu[k] += (double)ri.y * (double)ri.y;
+      vu[K + k] += (double)ri.x * (double)zi.x;
+      vu[K + k] += (double)ri.y * (double)zi.y;
+    }
+    if ((rows & 1) && tid == 0) {
+      const long long i = kb + rows - 1;
+      const TS di = fma(a, p[i], dv[i]);
+      const TS ri = fma(-a, q[i], rv[i]);
+      const TS zi = dinv[rows - 1] * ri;
+      dv[i] = di;
+      rv[i] = ri;
+      zv[i] = zi;
+      vu[k] += (double)ri * (double)ri;
+      vu[K + k] += (double)ri * (double)zi;
+    }
   }
   block_sum<2 * K>(vu, lds);
   if (tid == 0) {
@@ -876,11 +872,9 @@ template <typename TS, int K>
 __device__ void res_slab(const EngPtrs &P, int slab, int slot, const RveDev *R, bool need_cold,
                          double *lds)
 {
-  constexpr int KS = vec_stride(K);
-  const int tid = threadIdx.x;
+  const int tid = threadIdx.x, RS = 3 * P.SN;
   const int n = P.slab_node0[slab] + tid;
   const bool valid = n < R->nrow / 3;
-  const long long rbase = R->row_base;
   double v[4 * K];
 #pragma unroll
   for (int i = 0; i < 4 * K; i++) v[i] = 0.0;
@@ -911,50 +905,46 @@ __device__ void res_slab(const EngPtrs &P, int slab, int slot, const RveDev *R, 
         double vb[3][4];
 #pragma unroll
         for (int dd = 0; dd < 3; dd++) load_vec<double, 4>(D + t * kDictBlk + dd * 4, vb[dd]);
-        const double *xm = P.x64 + (rbase + 3LL * m) * KS;
+        double xv[3][K];
+        gather_node<double, K>(P.x64, node_vec(P, R->first_slab, m, K), RS, xv);
 #pragma unroll
-        for (int dd = 0; dd < 3; dd++) {
-          double xv[KS];
-          load_vec<double, KS>(xm + dd * KS, xv);
+        for (int dd = 0; dd < 3; dd++)
 #pragma unroll
           for (int d = 0; d < 3; d++)
 #pragma unroll
-            for (int k = 0; k < K; k++) r64[d][k] = fma(vb[dd][d], xv[k], r64[d][k]);
-          if (need_cold && nb_bc)
+            for (int k = 0; k < K; k++) r64[d][k] = fma(vb[dd][d], xv[dd][k], r64[d][k]);
+        if (need_cold && nb_bc)
+#pragma unroll
+          for (int dd = 0; dd < 3; dd++)
 #pragma unroll
             for (int d = 0; d < 3; d++)
 #pragma unroll
-              for (int k = 0; k < K; k++) rc64[d][k] = fma(vb[dd][d], xv[k], rc64[d][k]);
-        }
+              for (int k = 0; k < K; k++) rc64[d][k] = fma(vb[dd][d], xv[dd][k], rc64[d][k]);
       }
     }
-    TS *dv = (TS *)P.d, *rv = (TS *)P.r, *zv = (TS *)P.z;
-    const TS *dinv = (const TS *)P.dinv;
+    const long long nb = node_vec(P, R->first_slab, n, K);
+    const TS *dinv = (const TS *)P.dinv + (long long)slab * RS + 3 * tid;
+    TS ri[3][K], zi[3][K], zero[3][K];
 #pragma unroll
     for (int d = 0; d < 3; d++) {
-      const long long grow = rbase + 3LL * n + d;
-      TS ri[KS], zi[KS], zero[KS];
-      const TS dg = dinv[grow];
+      const TS dg = dinv[d];
 #pragma unroll
-      for (int k = 0; k < KS; k++) {
-        const int kk = k < K ? k : 0;
-        const double rk = (k < K) ? -r64[d][kk] : 0.0;
-        ri[k] = (TS)rk;
-        zi[k] = dg * ri[k];
-        zero[k] = (TS)0;
-        if (k < K) {
-          v[kk] += rk * rk;
-          v[K + kk] += (double)ri[k] * (double)ri[k];
-          v[2 * K + kk] += (double)ri[k] * (double)zi[k];
-          v[3 * K + kk] += rc64[d][kk] * rc64[d][kk];
-        }
+      for (int k = 0; k < K; k++) {
+        const double rk = -r64[d][k];
+        ri[d][k] = (TS)rk;
+        zi[d][k] = dg * ri[d][k];
+        zero[d][k] = (TS)0;
+        v[k] += rk * rk;
+        v[K + k] += (double)ri[d][k] * (double)ri[d][k];
+        v[2 * K + k] += (double)ri[d][k] * (double)zi[d][k];
+        v[3 * K + k] += rc64[d][k] * rc64[d][k];
       }
-      store_vec<TS, KS>(rv + grow * KS, ri);
-      store_vec<TS, KS>(zv + grow * KS, zi);
-      store_vec<TS, KS>(dv + grow * KS, zero);
-      store_vec<TS, KS>((TS *)P.p + grow * KS, zero);
-      store_vec<TS, KS>((TS *)P.q + grow * KS, zero);
     }
+    scatter_node<TS, K>((TS *)P.r, nb, RS, ri);
+    scatter_node<TS, K>((TS *)P.z, nb, RS, zi);
+    scatter_node<TS, K>((TS *)P.d, nb, RS, zero);
+    scatter_node<TS, K>((TS *)P.p, nb, RS, zero);
+    scatter_node<TS, K>((TS *)P.q, nb, RS, zero);
   }
   block_sum<4 * K>(v, lds);
   if (tid == 0) {
@@ -966,12 +956,10 @@ __device__ void res_slab(const EngPtrs &P, int slab, int slot, const RveDev *R, 
 template <typename TS, int K>
 __device__ void avg_slab(const EngPtrs &P, int slab, int slot, const RveDev *R, double *lds)
 {
-  constexpr int KS = vec_stride(K);
   __shared__ double bw[144];
   __shared__ double cm[kNmat * 36];
   const int tid = threadIdx.x;
   const int nslab = R->nslab;
-  const long long rbase = R->row_base;
   double *part = slab_part(P, slab);
   const int ls = slab - R->first_slab;
   const int nelm = R->nelm;
@@ -997,7 +985,7 @@ __device__ void avg_slab(const EngPtrs &P, int slab, int slot, const RveDev *R, 
         local_coords(a, &lx, &ly, &lz);
         const int node = n0 + lx + ly * R->nx + lz * R->nx * R->ny;
         for (int dd = 0; dd < 3; dd++) {
-          const double xv = P.x64[(rbase + 3 * node + dd) * KS + k];
+          const double xv = P.x64[node_vec(P, R->first_slab, node, K) + (long long)k * 3 * P.SN + dd];
 #pragma unroll
           for (int vv = 0; vv < 6; vv++) eps[vv] += bw[vv * 24 + 3 * a + dd] * xv;
         }
@@ -1041,22 +1029,15 @@ __global__ __launch_bounds__(256) void k_step_r(EngPtrs P, SolveParams sp)
 template <typename TS, int K, bool DICT>
 __global__ __launch_bounds__(256) void k_spmv_time(EngPtrs P)
 {
-  constexpr int KS = vec_stride(K);
   const int slab = xcd_slab(), tid = threadIdx.x;
   const int slot = P.slab_rve[slab];
   if (slot < 0) return;
   const RveDev *R = P.rves + slot;
   const int n = P.slab_node0[slab] + tid;
   if (n >= R->nrow / 3) return;
-  const long long rbase = R->row_base;
-  TS w[3][KS];
-  slab_node_spmv<TS, KS, DICT>(P, R, slot, slab, tid, n, w);
-  TS o[3 * KS];
-#pragma unroll
-  for (int d = 0; d < 3; d++)
-#pragma unroll
-    for (int k = 0; k < KS; k++) o[d * KS + k] = w[d][k];
-  store_vec<TS, 3 * KS>((TS *)P.q + (rbase + 3LL * n) * KS, o);
+  TS w[3][K];
+  slab_node_spmv<TS, K, DICT>(P, R, slot, slab, tid, n, 0u, w);
+  scatter_node<TS, K>((TS *)P.q, node_vec(P, R->first_slab, n, K), 3 * P.SN, w);
 }
 
 // ---------------------------------------------------------------------------
@@ -1067,7 +1048,7 @@ static int launch_steps(const EngPtrs &P, const SolveParams &sp, int grid, int n
 {
   if (grid % kXcd) return MM_ERR_ARG;  // the XCD-aware slab order needs whole groups of 8
   for (int i = 0; i < nsteps; i++) {
-    k_step_a<TS, K, DICT><<<grid, P.SN, 3 * P.SN * vec_stride(K) * sizeof(TS), s>>>(P, sp);
+    k_step_a<TS, K, DICT><<<grid, P.SN, 0, s>>>(P, sp);
     k_step_b<TS, K><<<grid, P.SN, 0, s>>>(P, sp);
     k_step_r<TS, K><<<grid, P.SN, 0, s>>>(P, sp);
   }

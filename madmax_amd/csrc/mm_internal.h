@@ -146,7 +146,8 @@ struct EngPtrs {
   const uint8_t *npat;  // [slab][SN] material pattern of the node's 8 elements
   const void *dict;     // [rve slot][256][27][12] interior 3-row ELL blocks (TS)
   const double *dict64; // the same in FP64 (the residual's matrix)
-  int SN;  // nodes per slab = lanes per workgroup
+  int SN;        // nodes per slab = lanes per workgroup (a power of two)
+  int sn_shift;  // log2(SN)
 };
 
 // Dictionary ELL: entry (slot, pattern) holds, for the 27 neighbour blocks t
@@ -174,6 +175,6 @@ int eng_launch_ke(int n, const int *work, const RveDev *rves, double *ke_tab, do
                   double *bw_tab, hipStream_t s);
 int eng_launch_phase(int nblk, const int2 *work, const RveDev *rves, uint8_t *elem_mat, hipStream_t s);
 constexpr int kPartDoubles = 96;
-inline constexpr int vec_stride_host(int K) { return (K == 1) ? 1 : ((K + 1) & ~1); }
+
 
 }  // namespace mm
