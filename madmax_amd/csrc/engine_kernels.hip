@@ -43,6 +43,13 @@ int upload_slot_tables(hipStream_t s)
 
 constexpr int kPart = kPartDoubles;
 
+// LDS scratch of the solver kernels: 16 doubles per reduced value, for the
+// largest reduction a kernel performs (4K residual norms, 12 averages).
+template <int K>
+struct RedLds {
+  static constexpr int size = 16 * ((4 * K > 12) ? 4 * K : 12);
+};
+
 
 // MI355X dispatches workgroup i to XCD i % 8.  The solver grids are a
 // multiple of 8 and workgroup i processes slab (i % 8) * (grid / 8) + i / 8,
@@ -622,7 +629,7 @@ __device__ bool inner_check(SolveState &N, const double (&s2)[2 * K], const Solv
 // 344-368, :80) from the averaging partials; run by one workgroup.
 template <int K>
 __device__ void finish_rve(const EngPtrs &P, int slot, const RveDev *R, SolveState &N,
-                           const SolveParams &sp, double *lds)
+                           const SolveParams &sp, double (&lds)[RedLds<K>::size])
 {
   __shared__ double tot[MM_MAX_RHS][12];
   const double *p0 = slab_part(P, R->first_slab);
@@ -687,7 +694,7 @@ __device__ void finish_rve(const EngPtrs &P, int slot, const RveDev *R, SolveSta
 template <typename TS, int K, bool DICT>
 __global__ __launch_bounds__(256) void k_step_a(EngPtrs P, SolveParams sp)
 {
-  __shared__ double lds[4 * K * 16];
+  __shared__ double lds[RedLds<K>::size];
   __shared__ SolveState N;
   __shared__ int s_xupd;
   const int slab = xcd_slab(), tid = threadIdx.x, SN = P.SN, RS = 3 * SN;
@@ -870,7 +877,7 @@ __global__ __launch_bounds__(256) void k_step_b(EngPtrs P, SolveParams sp)
 // sum_gp (.) w_gp per element through sum_gp B w, one load case at a time.
 template <typename TS, int K>
 __device__ void res_slab(const EngPtrs &P, int slab, int slot, const RveDev *R, bool need_cold,
-                         double *lds)
+                         double (&lds)[RedLds<K>::size])
 {
   const int tid = threadIdx.x, RS = 3 * P.SN;
   const int n = P.slab_node0[slab] + tid;
@@ -954,7 +961,8 @@ __device__ void res_slab(const EngPtrs &P, int slab, int slot, const RveDev *R, 
 }
 
 template <typename TS, int K>
-__device__ void avg_slab(const EngPtrs &P, int slab, int slot, const RveDev *R, double *lds)
+__device__ void avg_slab(const EngPtrs &P, int slab, int slot, const RveDev *R,
+                         double (&lds)[RedLds<K>::size])
 {
   __shared__ double bw[144];
   __shared__ double cm[kNmat * 36];
@@ -1007,7 +1015,7 @@ __device__ void avg_slab(const EngPtrs &P, int slab, int slot, const RveDev *R, 
 template <typename TS, int K>
 __global__ __launch_bounds__(256) void k_step_r(EngPtrs P, SolveParams sp)
 {
-  __shared__ double lds[4 * K * 16];
+  __shared__ double lds[RedLds<K>::size];
   const int slab = xcd_slab(), tid = threadIdx.x;
   const int slot = P.slab_rve[slab];
   if (slot < 0) return;

@@ -16,10 +16,12 @@ __device__ inline double load_coherent(const double *p)
 }
 
 // Sums NV values over the block in a fixed order; the result is valid in
-// thread 0.  lds must hold NV * 16 doubles (blockDim.x <= 1024).
-template <int NV>
-__device__ inline void block_sum(double (&v)[NV], double *lds)
+// thread 0.  lds holds 16 doubles per value (blockDim.x <= 1024); its size
+// is checked at compile time.
+template <int NV, int L>
+__device__ inline void block_sum(double (&v)[NV], double (&lds)[L])
 {
+  static_assert(L >= NV * 16, "block_sum: LDS scratch too small");
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1)
 #pragma unroll
@@ -69,9 +71,9 @@ __device__ inline bool arrive_last(int *counter, int nblocks)
 // partials (offset off) over the group's nslab partial rows (row stride
 // `stride`) in a fixed order (lane-strided, then block_sum).  The total is
 // valid in thread 0.
-template <int NV>
+template <int NV, int L>
 __device__ inline void group_reduce(const double *p0, int nslab, int stride, int off, double (&out)[NV],
-                                    double *lds)
+                                    double (&lds)[L])
 {
 #pragma unroll
   for (int i = 0; i < NV; i++) out[i] = 0.0;
@@ -84,9 +86,9 @@ __device__ inline void group_reduce(const double *p0, int nslab, int stride, int
 // Like group_reduce, for partials written by an earlier launch (visible at
 // kernel start, so plain cached loads).  Every workgroup that calls it over
 // the same partials obtains the same bits.
-template <int NV>
+template <int NV, int L>
 __device__ inline void group_sum(const double *p0, int nslab, int stride, int off, double (&out)[NV],
-                                 double *lds)
+                                 double (&lds)[L])
 {
 #pragma unroll
   for (int i = 0; i < NV; i++) out[i] = 0.0;

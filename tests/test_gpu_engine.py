@@ -10,7 +10,8 @@ import numpy as np
 import pytest
 
 from madmax_amd import (MM_ELL_DICT, MM_ELL_FULL, MM_PREC_FP64, MM_PREC_MIXED, MM_TANGENT_FD,
-                        MM_TANGENT_UNIT, Engine, configs, fd_loads, rve_spec, unit_loads)
+                        MM_TANGENT_NONE, MM_TANGENT_UNIT, Engine, configs, fd_loads, rve_spec,
+                        unit_loads)
 from madmax_amd import _lib
 from tests.helpers import normwise_rel, oracle_rve
 
@@ -214,3 +215,23 @@ def test_spmv_timing_runs(storage):
         _solve_batch(eng, [configs.c2_spec(3, i, n=12) for i in range(2)], key0=60)
         ms, nbytes = eng.time_spmv(reps=3)
     assert ms > 0 and nbytes > 0
+
+
+@pytest.mark.parametrize("precision", [MM_PREC_FP64, MM_PREC_MIXED])
+def test_single_load_sequential_reuse(precision):
+    """One load case at a time on a resident RVE (nrhs 1, host check after
+    every step): the plain single-RVE path."""
+    spec = rve_spec(6, [(0.5, 0.5, 0.3)])
+    ref, _ = oracle_rve(spec)
+    with Engine(nrhs=1, precision=precision, hbm_budget=GB // 4, max_rves=2, slab_nodes=128,
+                inner_rtol=1e-10 if precision == MM_PREC_FP64 else 1e-5, rtol=1e-11,
+                max_sweeps=4 if precision == MM_PREC_FP64 else 8, sync_every_step=True,
+                use_graph=False) as eng:
+        for i, load in enumerate(unit_loads()):
+            assert eng.submit(i + 1, spec, [load], MM_TANGENT_NONE, warm_key=1) == 0
+            eng.drain()
+            r = eng.poll()[0]
+            out = ref.homog(load, rtol=1e-12)
+            print("load", i, r["status"], r["pcg_its"], r["stress"][0], out["stress_ave"], flush=True)
+            assert r["status"] == 0
+            assert normwise_rel(r["stress"][0], out["stress_ave"]) <= TOL
