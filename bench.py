@@ -6,11 +6,18 @@ six unit-strain load cases solved as a 6-RHS block by FP32 Jacobi-PCG inside
 FP64 iterative refinement, and their 6x6 tangents averaged in FP64.
 
 The JSON line also carries:
-  roofline      the batched ELL SpMV kernel, timed with HIP events
+  roofline      the solve's dominant kernel (the PCG step's SpMV + p/q update,
+                dictionary ELL storage), timed with HIP events on the engine
+                stream over a stand-alone replica of its work
+  spmv          the ELL SpMV alone: dictionary storage and the full
+                per-row ELL (the reference's layout) on the same batch
   cpu_baseline  the FP64 oracle (one core, one RVE at a time) on a sample
   plain_fp64    the plain FP64 single-RVE GPU path (one load case at a time)
   accuracy      max normwise relative tangent error vs the FP64 oracle
   serving       p50 / p99 latency of a short FE^2 request stream
+
+--kernels-only admits and solves the batch once, then runs the timing
+replicas and the HBM copy probe (the workload of the counter runs).
 """
 import argparse
 import json
@@ -24,6 +31,7 @@ sys.path.insert(0, ROOT)
 import numpy as np  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "pmc_k_inner_time_128x30cube.md")
 METRIC = "RVE tangents/sec at 30³ hex8 (rel err≤1e-8); p99 latency; SpMV HBM GB/s"
 
 
@@ -42,6 +50,18 @@ def run_batch(eng, specs, mm, warm=False):
     return [out[i + 1] for i in range(len(specs))]
 
 
+def measured_traffic(batch, n):
+    """HBM bytes per launch of the step kernel from the committed counter runs
+    (FETCH_SIZE + WRITE_SIZE, calibrated on the copy kernel), if they were
+    taken on this workload."""
+    if batch != 128 or n != 30 or not os.path.exists(TRAFFIC_FILE):
+        return None
+    for line in open(TRAFFIC_FILE):
+        if line.startswith("traffic_bytes_per_launch:"):
+            return float(line.split(":", 1)[1])
+    return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -58,6 +78,8 @@ def main():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--serve", type=int, default=1024, help="requests in the serving sample")
+    ap.add_argument("--no-full-ell", action="store_true", help="skip the full-ELL SpMV figure")
+    ap.add_argument("--kernels-only", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU; multi-GPU runs are not part of it "
@@ -86,7 +108,19 @@ def main():
     log("batch %d x %d^3 (%d rows each), arena %.1f GB" % (B, n, rows, budget / 1e9))
     eng = mm.Engine(nrhs=6, precision=mm.MM_PREC_MIXED, hbm_budget=budget, max_rves=B + 8,
                     slab_nodes=args.slab, inner_rtol=args.inner_rtol, rtol=args.rtol,
-                    steps_per_launch=args.spl, use_graph=True)
+                    steps_per_launch=args.spl, use_graph=True, ell_storage=mm.MM_ELL_DICT)
+    if args.kernels_only:
+        import ctypes
+        run_batch(eng, specs, mm, warm=True)
+        step_ms, step_bytes = eng.time_step(reps=20)
+        spmv_ms, spmv_bytes = eng.time_spmv(reps=20)
+        eng.close()
+        copy_ms = ctypes.c_double()
+        mm.lib().mm_hbm_copy_probe(0, 1 << 30, 5, ctypes.byref(copy_ms))
+        print(json.dumps({"step_ms": step_ms, "step_bytes": step_bytes, "spmv_ms": spmv_ms,
+                          "spmv_bytes": spmv_bytes, "copy_bytes_each_way": 1 << 30,
+                          "copy_ms": copy_ms.value}), flush=True)
+        return
     for w in range(args.warmup):
         t = time.perf_counter()
         run_batch(eng, specs, mm)
@@ -104,12 +138,34 @@ def main():
     log("timed: %.3f s for %d steps -> %.1f tangents/s (its %d..%d, sweeps %d..%d)"
         % (elapsed, args.steps, value, min(its), max(its), min(sweeps), max(sweeps)))
 
-    # roofline of the batched ELL SpMV on the resident batch
+    # roofline of the dominant kernel (the PCG step's SpMV + update) and the
+    # SpMV alone, on the resident batch
     run_batch(eng, specs, mm, warm=True)
+    step_ms, step_bytes = eng.time_step(reps=50)
+    achieved = step_bytes / (step_ms * 1e-3) / 1e9
+    log("step kernel: %.3f ms, %.3f GB/launch -> %.0f GB/s" % (step_ms, step_bytes / 1e9, achieved))
     spmv_ms, spmv_bytes = eng.time_spmv(reps=50)
-    achieved = spmv_bytes / (spmv_ms * 1e-3) / 1e9
-    log("spmv: %.3f ms, %.3f GB/launch -> %.0f GB/s" % (spmv_ms, spmv_bytes / 1e9, achieved))
+    spmv_gbs = spmv_bytes / (spmv_ms * 1e-3) / 1e9
+    log("spmv (dictionary ELL): %.3f ms, %.3f GB/launch -> %.0f GB/s" % (spmv_ms, spmv_bytes / 1e9,
+                                                                         spmv_gbs))
     eng.close()
+    full = None
+    if not args.no_full_ell:
+        fb = int((rows * (81 * 4 + 27 * 4 / 3.0) + per_rve) * B * 1.3) + (256 << 20)
+        fe = mm.Engine(nrhs=6, precision=mm.MM_PREC_MIXED, hbm_budget=fb, max_rves=B + 8,
+                       slab_nodes=args.slab, inner_rtol=args.inner_rtol, rtol=args.rtol,
+                       steps_per_launch=args.spl, use_graph=True, ell_storage=mm.MM_ELL_FULL)
+        t = time.perf_counter()
+        fres = run_batch(fe, specs, mm, warm=True)
+        tf = time.perf_counter() - t
+        f_ms, f_bytes = fe.time_spmv(reps=50)
+        fe.close()
+        same = all(np.array_equal(a["c_tangent"], b["c_tangent"]) for a, b in zip(fres, results))
+        full = dict(ms=f_ms, bytes=f_bytes, gbs=f_bytes / (f_ms * 1e-3) / 1e9,
+                    frac=f_bytes / (f_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                    solve_tangents_per_s=B / tf, tangents_equal_to_dictionary_run=same)
+        log("spmv (full ELL): %.3f ms, %.3f GB/launch -> %.0f GB/s; batch in %.3f s, same tangents: %s"
+            % (f_ms, f_bytes / 1e9, full["gbs"], tf, same))
     import ctypes
     copy_ms = ctypes.c_double()
     copy_bytes = 4 << 30
@@ -208,10 +264,14 @@ def main():
                    "batch": B, "slab_nodes": args.slab, "inner_rtol": args.inner_rtol,
                    "rtol": args.rtol, "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "k_spmv_time<float,6> (batched ELL SpMV, %d resident RVEs)" % B,
-                     "avg_ms": spmv_ms, "bytes_per_launch": spmv_bytes,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(B, n),
+                     "kernel": "k_inner_time<float,6,true>: the PCG step kernel's SpMV + p/q update "
+                               "(dictionary ELL), %d resident RVEs" % B,
+                     "avg_ms": step_ms, "bytes_per_launch": step_bytes,
                      "copy_gbs_measured": copy_gbs},
+        "spmv": {"dictionary": {"ms": spmv_ms, "bytes": spmv_bytes, "gbs": spmv_gbs,
+                                "frac": spmv_gbs / HBM_PEAK_GBS},
+                 "full_ell": full},
         "cpu_baseline": cpu,
         "plain_fp64_gpu": plain,
         "speedup_vs_plain": (value / plain["value"]) if plain else None,

@@ -232,6 +232,11 @@ int mm_engine_download_x(mm_engine *eng, uint64_t warm_key, int load, double *x)
  * RVEs with HIP events on the engine stream; writes the average launch
  * duration and the algorithmic bytes of one launch. */
 int mm_engine_time_spmv(mm_engine *eng, int reps, double *avg_ms, double *bytes);
+/* The same for the PCG step's SpMV kernel as it runs in the solve: SpMV plus
+ * the p = z + beta p, q = w + beta q update and the p.q partials (the solve's
+ * dominant kernel).  Both overwrite solver vectors: only with no RVE being
+ * solved (MM_ERR_STATE otherwise). */
+int mm_engine_time_step(mm_engine *eng, int reps, double *avg_ms, double *bytes);
 
 /* Copies `bytes` between two device buffers `reps` times (the runtime's
  * device-to-device copy kernel) and reports the average duration: the

@@ -154,6 +154,7 @@ SIGNATURES = {
     "mm_engine_download_phase": (C.c_int, [C.c_void_p, C.c_uint64, ip]),
     "mm_engine_download_x": (C.c_int, [C.c_void_p, C.c_uint64, C.c_int, dp]),
     "mm_engine_time_spmv": (C.c_int, [C.c_void_p, C.c_int, dp, dp]),
+    "mm_engine_time_step": (C.c_int, [C.c_void_p, C.c_int, dp, dp]),
     "mm_hbm_copy_probe": (C.c_int, [C.c_int, C.c_size_t, C.c_int, dp]),
 }
 

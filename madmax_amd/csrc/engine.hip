@@ -1015,18 +1015,21 @@ extern "C" int mm_engine_download_x(mm_engine *e, uint64_t key, int load, double
   return 0;
 }
 
-extern "C" int mm_engine_time_spmv(mm_engine *e, int reps, double *avg_ms, double *bytes)
+// Times `reps` launches of a stand-alone solver kernel over the resident RVEs
+// with HIP events on the engine stream.  what 0: the SpMV w = A z; what 1:
+// kernel A's INNER work (SpMV and the p, q update).  Algorithmic bytes of one
+// launch: z read once per row and load case, w written (what 0) or p and q
+// read and written (what 1); full storage adds every row's 81 values and
+// every node's 27 block columns, dictionary storage one pattern byte per node
+// and each resident RVE's dictionary once.
+static int time_kernel(mm_engine *e, int what, int reps, double *avg_ms, double *bytes)
 {
   if (!e || reps < 1) return MM_ERR_ARG;
   std::lock_guard<std::mutex> lk(e->step_mu);
   long long hw = e->slabs.high_water();
-  if (hw <= 0 || e->n_active > 0) return MM_ERR_STATE;  // it overwrites q
+  if (hw <= 0 || e->n_active > 0) return MM_ERR_STATE;  // it overwrites solver vectors
   const int grid = solver_grid(e, hw);
   EngPtrs P = e->ptrs();
-  // algorithmic bytes of one launch: z read and w written once per row and
-  // load case; full storage: every row's 81 values and every node's 27 block
-  // columns; dictionary storage: one pattern byte per node and each resident
-  // RVE's dictionary once
   double rows = 0.0, nrve = 0.0;
   for (int s = 0; s < e->rve_cap; s++)
     if (e->slots[s].state != SL_FREE) {
@@ -1036,11 +1039,11 @@ extern "C" int mm_engine_time_spmv(mm_engine *e, int reps, double *avg_ms, doubl
   hipEvent_t a, b;
   MM_HIP(hipEventCreate(&a));
   MM_HIP(hipEventCreate(&b));
-  int rc = eng_launch_time(e->fp64, e->K, e->dict_mode, P, grid, e->stream);  // warm-up
+  int rc = eng_launch_time(e->fp64, e->K, e->dict_mode, P, grid, what, e->stream);  // warm-up
   if (rc) return rc;
   MM_HIP(hipEventRecord(a, e->stream));
   for (int i = 0; i < reps; i++) {
-    rc = eng_launch_time(e->fp64, e->K, e->dict_mode, P, grid, e->stream);
+    rc = eng_launch_time(e->fp64, e->K, e->dict_mode, P, grid, what, e->stream);
     if (rc) return rc;
   }
   MM_HIP(hipEventRecord(b, e->stream));
@@ -1049,14 +1052,24 @@ extern "C" int mm_engine_time_spmv(mm_engine *e, int reps, double *avg_ms, doubl
   MM_HIP(hipEventElapsedTime(&ms, a, b));
   (void)hipEventDestroy(a);
   (void)hipEventDestroy(b);
-  e->spmv_ms = ms / reps;
-  if (avg_ms) *avg_ms = e->spmv_ms;
+  if (what == 0) e->spmv_ms = ms / reps;
+  if (avg_ms) *avg_ms = ms / reps;
   if (bytes) {
-    const double vec = rows * 2.0 * e->K * e->tsz;
+    const double vec = rows * (what == 0 ? 2.0 : 5.0) * e->K * e->tsz;
     *bytes = e->dict_mode ? vec + rows / 3.0 + nrve * kDictPat * kDictEntry * (double)e->tsz
                           : vec + rows * (kW * (double)e->tsz + 27.0 * 4.0 / 3.0);
   }
   return 0;
+}
+
+extern "C" int mm_engine_time_spmv(mm_engine *e, int reps, double *avg_ms, double *bytes)
+{
+  return time_kernel(e, 0, reps, avg_ms, bytes);
+}
+
+extern "C" int mm_engine_time_step(mm_engine *e, int reps, double *avg_ms, double *bytes)
+{
+  return time_kernel(e, 1, reps, avg_ms, bytes);
 }
 
 extern "C" int mm_hbm_copy_probe(int device, size_t bytes, int reps, double *avg_ms)

@@ -434,12 +434,12 @@ __device__ __forceinline__ void scatter_node(T *__restrict__ x, long long nb, in
     for (int d = 0; d < 3; d++) x[nb + (long long)k * RS + d] = v[d][k];
 }
 
-// Row block of lane tid (node n) times z, full storage: the 27 slot blocks in
-// the reference's slot order, w[d] accumulated over slots j = 3t + d'.
-// Load cases set in `skip` are not needed and are left out.
+// Row block of lane tid (node n) times z, full storage: the 27 slot blocks
+// visited in the reference's slot order, w[d] accumulated over slots
+// j = 3t + d'.  zc receives the node's own z rows.
 template <typename TS, int K>
-__device__ __forceinline__ void node_spmv(const EngPtrs &P, int first_slab, int slab, int tid, unsigned skip,
-                                          const TS *__restrict__ z, TS (&w)[3][K])
+__device__ __forceinline__ void node_spmv(const EngPtrs &P, int first_slab, int slab, int tid, int n,
+                                          const TS *__restrict__ z, TS (&w)[3][K], TS (&zc)[3][K])
 {
   const int SN = P.SN, RS = 3 * SN;
   const int *__restrict__ bcols = P.bcols;
@@ -462,7 +462,6 @@ __device__ __forceinline__ void node_spmv(const EngPtrs &P, int first_slab, int 
       const long long nb = node_vec(P, first_slab, m, K);
 #pragma unroll
       for (int k = 0; k < K; k++) {
-        if ((skip >> k) & 1u) continue;
         TS x[3];
 #pragma unroll
         for (int dd = 0; dd < 3; dd++) x[dd] = z[nb + (long long)k * RS + dd];
@@ -473,49 +472,46 @@ __device__ __forceinline__ void node_spmv(const EngPtrs &P, int first_slab, int 
       }
     }
   }
+  gather_node<TS, K>(z, node_vec(P, first_slab, n, K), RS, zc);
 }
 
-// The same product from the dictionary: an interior node's 27 blocks come
-// from entry npat of its RVE (lanes of one phase read the same addresses, so
-// the values stay in L1/L2) at the structured neighbour offsets; a boundary
-// node's row is the identity, whose full-ELL product z + 0 is reproduced
-// exactly.  Same fma order as node_spmv, so the same bits.
-template <typename TS, int K>
-__device__ __forceinline__ void node_spmv_dict(const EngPtrs &P, const RveDev *R, int slot, int slab, int tid,
-                                               int n, unsigned skip, const TS *__restrict__ z, TS (&w)[3][K])
+// Reads an address that is the same in every lane through the scalar unit.
+template <typename T>
+__device__ __forceinline__ T uniform_load(const T *p)
+{
+  return *(const __attribute__((address_space(4))) T *)p;
+}
+
+// The 27-block product of an interior node from dictionary entry D; UNI: D is
+// the same in every lane and its values are read through the scalar unit.
+template <typename TS, int K, bool UNI>
+__device__ __forceinline__ void dict_rows(const EngPtrs &P, const TS *D, int first_slab, int n, int nx, int nxy,
+                                          const TS *__restrict__ z, TS (&w)[3][K], TS (&zc)[3][K])
 {
   const int RS = 3 * P.SN;
-  const int nx = R->nx, nxy = R->nx * R->ny;
-  const int ix = n % nx, iy = (n / nx) % R->ny, iz = n / nxy;
-  const bool bc = node_class(ix, iy, iz, nx, R->ny, R->nz) != 13;
-  if (bc) {
-    const long long nb = node_vec(P, R->first_slab, n, K);
-#pragma unroll
-    for (int k = 0; k < K; k++)
-#pragma unroll
-      for (int d = 0; d < 3; d++) w[d][k] = ((skip >> k) & 1u) ? (TS)0 : z[nb + (long long)k * RS + d] + (TS)0;
-    return;
-  }
-#pragma unroll
-  for (int d = 0; d < 3; d++)
-#pragma unroll
-    for (int k = 0; k < K; k++) w[d][k] = (TS)0;
-  const TS *D = (const TS *)P.dict +
-                ((size_t)slot * kDictPat + P.npat[(long long)slab * P.SN + tid]) * kDictEntry;
 #pragma unroll 3
   for (int t = 0; t < 27; t++) {
     const int key = g_slots.off[13][t];
     const int m = n + (key % 3 - 1) + ((key / 3) % 3 - 1) * nx + (key / 9 - 1) * nxy;
     TS v[3][4];  // [d'][d]
+    if (UNI) {
 #pragma unroll
-    for (int dd = 0; dd < 3; dd++) load_vec<TS, 4>(D + t * kDictBlk + dd * 4, v[dd]);
-    const long long nb = node_vec(P, R->first_slab, m, K);
+      for (int dd = 0; dd < 3; dd++)
+#pragma unroll
+        for (int d = 0; d < 3; d++) v[dd][d] = uniform_load(D + t * kDictBlk + dd * 4 + d);
+    } else {
+#pragma unroll
+      for (int dd = 0; dd < 3; dd++) load_vec<TS, 4>(D + t * kDictBlk + dd * 4, v[dd]);
+    }
+    const long long nb = node_vec(P, first_slab, m, K);
 #pragma unroll
     for (int k = 0; k < K; k++) {
-      if ((skip >> k) & 1u) continue;
       TS x[3];
 #pragma unroll
       for (int dd = 0; dd < 3; dd++) x[dd] = z[nb + (long long)k * RS + dd];
+      if (key == 13)
+#pragma unroll
+        for (int dd = 0; dd < 3; dd++) zc[dd][k] = x[dd];
 #pragma unroll
       for (int dd = 0; dd < 3; dd++)
 #pragma unroll
@@ -524,12 +520,53 @@ __device__ __forceinline__ void node_spmv_dict(const EngPtrs &P, const RveDev *R
   }
 }
 
+// The same product from the dictionary: an interior node's 27 blocks come
+// from entry npat of its RVE at the structured neighbour offsets; a boundary
+// node's row is the identity, whose full-ELL product z + 0 is reproduced
+// exactly.  Same fma order as node_spmv, so the same bits.  When every
+// interior lane of the wavefront has the same pattern (the common case) the
+// block values are read once per wavefront through the scalar unit instead of
+// once per lane through L1.  zc receives the node's own z rows (the centre
+// block of the gather).
+template <typename TS, int K>
+__device__ __forceinline__ void node_spmv_dict(const EngPtrs &P, const RveDev *R, int slot, int slab, int tid,
+                                               int n, const TS *__restrict__ z, TS (&w)[3][K],
+                                               TS (&zc)[3][K])
+{
+  const int RS = 3 * P.SN;
+  const int nx = R->nx, nxy = R->nx * R->ny;
+  const int ix = n % nx, iy = (n / nx) % R->ny, iz = n / nxy;
+  const bool bc = node_class(ix, iy, iz, nx, R->ny, R->nz) != 13;
+  if (bc) {
+    gather_node<TS, K>(z, node_vec(P, R->first_slab, n, K), RS, zc);
+#pragma unroll
+    for (int k = 0; k < K; k++)
+#pragma unroll
+      for (int d = 0; d < 3; d++) w[d][k] = zc[d][k] + (TS)0;
+    return;
+  }
+#pragma unroll
+  for (int d = 0; d < 3; d++)
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+      w[d][k] = (TS)0;
+      zc[d][k] = (TS)0;
+    }
+  const TS *D0 = (const TS *)P.dict + (size_t)slot * kDictPat * kDictEntry;
+  const int pat = P.npat[(long long)slab * P.SN + tid];
+  const int pat0 = __builtin_amdgcn_readfirstlane(pat);
+  if (__all(pat == pat0))
+    dict_rows<TS, K, true>(P, D0 + (size_t)pat0 * kDictEntry, R->first_slab, n, nx, nxy, z, w, zc);
+  else
+    dict_rows<TS, K, false>(P, D0 + (size_t)pat * kDictEntry, R->first_slab, n, nx, nxy, z, w, zc);
+}
+
 template <typename TS, int K, bool DICT>
 __device__ __forceinline__ void slab_node_spmv(const EngPtrs &P, const RveDev *R, int slot, int slab, int tid,
-                                               int n, unsigned skip, TS (&w)[3][K])
+                                               int n, TS (&w)[3][K], TS (&zc)[3][K])
 {
-  if (DICT) node_spmv_dict<TS, K>(P, R, slot, slab, tid, n, skip, (const TS *)P.z, w);
-  else node_spmv<TS, K>(P, R->first_slab, slab, tid, skip, (const TS *)P.z, w);
+  if (DICT) node_spmv_dict<TS, K>(P, R, slot, slab, tid, n, (const TS *)P.z, w, zc);
+  else node_spmv<TS, K>(P, R->first_slab, slab, tid, n, (const TS *)P.z, w, zc);
 }
 
 // Element-wise passes: lane tid handles the value pairs tid, tid + blockDim,
@@ -682,6 +719,50 @@ __device__ void finish_rve(const EngPtrs &P, int slot, const RveDev *R, SolveSta
   __threadfence_system();
 }
 
+// The INNER work of kernel A for one slab: w = A z (one lane per node), then
+// on the node's own rows p = z + beta p, q = w + beta q (so q = A p without
+// gathering p) and the partial p.q.  Every load case is gathered (a per-case
+// branch between the gathers would serialize their loads); the cases in
+// `skip` have met their sweep target and skip only the update.  The own z
+// rows come from the gather, not a second read.
+template <typename TS, int K, bool DICT>
+__device__ __forceinline__ void inner_a(const EngPtrs &P, const RveDev *R, int slot, int slab, unsigned skip,
+                                        const double *beta, double (&lds)[RedLds<K>::size])
+{
+  const int tid = threadIdx.x, RS = 3 * P.SN;
+  const int n = P.slab_node0[slab] + tid;
+  const long long sb = (long long)slab * K * RS;
+  double v[K];
+#pragma unroll
+  for (int k = 0; k < K; k++) v[k] = 0.0;
+  if (n < R->nrow / 3) {
+    TS w[3][K], zc[3][K];
+    slab_node_spmv<TS, K, DICT>(P, R, slot, slab, tid, n, w, zc);
+    TS *p = (TS *)P.p;
+    TS *q = (TS *)P.q;
+    const long long nb = sb + 3 * tid;
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+      if ((skip >> k) & 1u) continue;
+      const TS b = (TS)beta[k];
+      const long long i = nb + (long long)k * RS;
+#pragma unroll
+      for (int d = 0; d < 3; d++) {
+        const TS pi = fma(b, p[i + d], zc[d][k]);
+        const TS qi = fma(b, q[i + d], w[d][k]);
+        p[i + d] = pi;
+        q[i + d] = qi;
+        v[k] += (double)pi * (double)qi;
+      }
+    }
+  }
+  block_sum<K>(v, lds);
+  if (tid == 0) {
+    double *part = slab_part(P, slab);
+    for (int k = 0; k < K; k++) part[k] = v[k];
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Kernel A (reads st[2], writes st[0]).
 // PH_RESRED: starts the sweep from the residual norms, then continues as
@@ -745,40 +826,7 @@ __global__ __launch_bounds__(256) void k_step_a(EngPtrs P, SolveParams sp)
   }
 
   // PH_INNER
-  const unsigned skip = N.done_in;
-  const int n = n0 + tid;
-  double v[K];
-#pragma unroll
-  for (int k = 0; k < K; k++) v[k] = 0.0;
-  if (n < R->nrow / 3) {
-    // every load case is gathered (a per-case branch between the gathers
-    // would serialize their loads); finished cases skip only the update
-    TS w[3][K];
-    slab_node_spmv<TS, K, DICT>(P, R, slot, slab, tid, n, 0u, w);
-    TS *p = (TS *)P.p;
-    TS *q = (TS *)P.q;
-    const TS *z = (const TS *)P.z;
-    const long long nb = sb + 3 * tid;
-#pragma unroll
-    for (int k = 0; k < K; k++) {
-      if ((skip >> k) & 1u) continue;
-      const TS b = (TS)N.beta[k];
-      const long long i = nb + (long long)k * RS;
-#pragma unroll
-      for (int d = 0; d < 3; d++) {
-        const TS pi = fma(b, p[i + d], z[i + d]);
-        const TS qi = fma(b, q[i + d], w[d][k]);
-        p[i + d] = pi;
-        q[i + d] = qi;
-        v[k] += (double)pi * (double)qi;
-      }
-    }
-  }
-  block_sum<K>(v, lds);
-  if (tid == 0) {
-    double *part = slab_part(P, slab);
-    for (int k = 0; k < K; k++) part[k] = v[k];
-  }
+  inner_a<TS, K, DICT>(P, R, slot, slab, N.done_in, N.beta, lds);
 }
 
 // ---------------------------------------------------------------------------
@@ -1043,9 +1091,25 @@ __global__ __launch_bounds__(256) void k_spmv_time(EngPtrs P)
   const RveDev *R = P.rves + slot;
   const int n = P.slab_node0[slab] + tid;
   if (n >= R->nrow / 3) return;
-  TS w[3][K];
-  slab_node_spmv<TS, K, DICT>(P, R, slot, slab, tid, n, 0u, w);
+  TS w[3][K], zc[3][K];
+  slab_node_spmv<TS, K, DICT>(P, R, slot, slab, tid, n, w, zc);
   scatter_node<TS, K>((TS *)P.q, node_vec(P, R->first_slab, n, K), 3 * P.SN, w);
+}
+
+// Stand-alone replica of kernel A's INNER work (inner_a, beta = 1/2 for every
+// load case) over every resident RVE; overwrites p, q and the partials, so it
+// is used only when no RVE is being solved.
+template <typename TS, int K, bool DICT>
+__global__ __launch_bounds__(256) void k_inner_time(EngPtrs P)
+{
+  __shared__ double lds[RedLds<K>::size];
+  __shared__ double beta[K];
+  const int slab = xcd_slab(), tid = threadIdx.x;
+  const int slot = P.slab_rve[slab];
+  if (slot < 0) return;
+  if (tid < K) beta[tid] = 0.5;
+  __syncthreads();
+  inner_a<TS, K, DICT>(P, P.rves + slot, slot, slab, 0u, beta, lds);
 }
 
 // ---------------------------------------------------------------------------
@@ -1077,9 +1141,10 @@ static int launch_assemble(const EngPtrs &P, int nblk, const int *slab_list, int
 }
 
 template <typename TS, int K, bool DICT>
-static int launch_time(const EngPtrs &P, int grid, hipStream_t s)
+static int launch_time(const EngPtrs &P, int grid, int what, hipStream_t s)
 {
-  k_spmv_time<TS, K, DICT><<<grid, P.SN, 0, s>>>(P);
+  if (what == 1) k_inner_time<TS, K, DICT><<<grid, P.SN, 0, s>>>(P);
+  else k_spmv_time<TS, K, DICT><<<grid, P.SN, 0, s>>>(P);
   MM_HIP(hipGetLastError());
   return 0;
 }
@@ -1135,14 +1200,14 @@ int eng_launch_assemble(bool fp64, int K, const EngPtrs &P, int nblk, const int 
 #undef CALL_ASM
 }
 
-int eng_launch_time(bool fp64, int K, bool dict, const EngPtrs &P, int grid, hipStream_t s)
+int eng_launch_time(bool fp64, int K, bool dict, const EngPtrs &P, int grid, int what, hipStream_t s)
 {
   if (dict) {
-#define CALL_TIME(T, KK) launch_time<T, KK, true>(P, grid, s)
+#define CALL_TIME(T, KK) launch_time<T, KK, true>(P, grid, what, s)
     MM_DISPATCH(fp64, K, CALL_TIME);
 #undef CALL_TIME
   }
-#define CALL_TIME(T, KK) launch_time<T, KK, false>(P, grid, s)
+#define CALL_TIME(T, KK) launch_time<T, KK, false>(P, grid, what, s)
   MM_DISPATCH(fp64, K, CALL_TIME);
 #undef CALL_TIME
 }

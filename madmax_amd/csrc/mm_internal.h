@@ -168,7 +168,8 @@ int eng_launch_steps(bool fp64, int K, bool dict, const EngPtrs &P, const SolveP
 int eng_launch_dict(bool fp64, int n, const int *slots, const double *ke_tab, void *dict, hipStream_t s);
 int eng_launch_assemble(bool fp64, int K, const EngPtrs &P, int nblk, const int *slab_list, int mode,
                         hipStream_t s);
-int eng_launch_time(bool fp64, int K, bool dict, const EngPtrs &P, int grid, hipStream_t s);
+// what: 0 = the SpMV w = A z alone (k_spmv_time), 1 = kernel A's INNER work
+int eng_launch_time(bool fp64, int K, bool dict, const EngPtrs &P, int grid, int what, hipStream_t s);
 int eng_launch_admit(int n, const int *slots, const RveDev *states, RveDev *rves, int *slab_rve,
                      int *slab_node0, int SN, hipStream_t s);
 int eng_launch_ke(int n, const int *work, const RveDev *rves, double *ke_tab, double *c_tab_out,

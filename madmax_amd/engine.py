@@ -192,6 +192,13 @@ class Engine:
         check(lib().mm_engine_time_spmv(self.h, reps, C.byref(ms), C.byref(by)), "time_spmv")
         return ms.value, by.value
 
+    def time_step(self, reps=20):
+        """Average duration (ms) and algorithmic bytes of the PCG step's SpMV
+        kernel (SpMV + p/q update) over the resident RVEs."""
+        ms, by = C.c_double(), C.c_double()
+        check(lib().mm_engine_time_step(self.h, reps, C.byref(ms), C.byref(by)), "time_step")
+        return ms.value, by.value
+
 
 class Micro:
     """The reference's process-global micro (micro_init + homog_* entry points)."""

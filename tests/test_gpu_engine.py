@@ -214,7 +214,9 @@ def test_spmv_timing_runs(storage):
     with Engine(nrhs=6, hbm_budget=GB, max_rves=8, ell_storage=storage) as eng:
         _solve_batch(eng, [configs.c2_spec(3, i, n=12) for i in range(2)], key0=60)
         ms, nbytes = eng.time_spmv(reps=3)
+        ms2, nbytes2 = eng.time_step(reps=3)
     assert ms > 0 and nbytes > 0
+    assert ms2 > 0 and nbytes2 > nbytes
 
 
 @pytest.mark.parametrize("precision", [MM_PREC_FP64, MM_PREC_MIXED])
