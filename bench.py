@@ -18,6 +18,15 @@ The JSON line also carries:
 
 --kernels-only admits and solves the batch once, then runs the timing
 replicas and the HBM copy probe (the workload of the counter runs).
+
+--workload selects other seeded workloads, each printing its own JSON line:
+  c3  one 100^3 RVE (3.09 M dofs, 4 fibers, random strain): solve time,
+      PCG iterations, refinement sweeps, SpMV GB/s of both storages, and the
+      mixed-precision stress against an FP64 solve
+  c4  512 RVEs of uneven size (8^3..32^3), 6 load cases each: tangents/s
+  c5  FE^2 serving stream (12^3/20^3/30^3 RVEs, strain + tangent per request,
+      Poisson arrivals), two Newton passes over the same gauss points, the
+      second warm-started from the resident RVEs: p50 / p99 latency
 """
 import argparse
 import json
@@ -62,6 +71,97 @@ def measured_traffic(batch, n):
     return None
 
 
+def workload_c3(args, mm, configs):
+    """One 100^3 RVE, one random strain."""
+    spec = configs.c3_spec()
+    eps = configs.c3_strain()
+    rows = 3 * 101 ** 3
+    out = {"workload": "c3: one 100^3 hex8 RVE (%d dofs), 4 fibers vf 0.4, strain ~ N(0, 1e-3)" % rows}
+    res = {}
+    for name, prec, storage in (("mixed_dict", mm.MM_PREC_MIXED, mm.MM_ELL_DICT),
+                                ("mixed_full", mm.MM_PREC_MIXED, mm.MM_ELL_FULL),
+                                ("fp64_dict", mm.MM_PREC_FP64, mm.MM_ELL_DICT)):
+        budget = int(rows * (81 * 8 + 200) * 1.3) + (512 << 20)
+        with mm.Engine(nrhs=1, precision=prec, hbm_budget=budget, max_rves=2, slab_nodes=256,
+                       rtol=1e-10, inner_rtol=1e-5 if prec == mm.MM_PREC_MIXED else 1e-10,
+                       max_sweeps=12 if prec == mm.MM_PREC_MIXED else 2, max_inner_its=200000,
+                       ell_storage=storage) as eng:
+            t = time.perf_counter()
+            eng.submit(1, spec, [eps], mm.MM_TANGENT_NONE, warm_key=1)
+            eng.drain()
+            r = eng.poll()[0]
+            dt = time.perf_counter() - t
+            ms, nb = eng.time_spmv(reps=20)
+            sms, sb = eng.time_step(reps=20)
+        res[name] = r
+        out[name] = dict(seconds=dt, status=r["status"], pcg_its=r["pcg_its"], sweeps=r["sweeps"],
+                         residual_rel=r["residual_rel"], spmv_ms=ms, spmv_gbs=nb / (ms * 1e-3) / 1e9,
+                         spmv_frac=nb / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, step_ms=sms,
+                         step_gbs=sb / (sms * 1e-3) / 1e9)
+        log("c3 %s: %.2f s, %d its, %d sweeps, SpMV %.3f ms -> %.0f GB/s"
+            % (name, dt, r["pcg_its"], r["sweeps"], ms, out[name]["spmv_gbs"]))
+    a, b = res["mixed_dict"]["stress"][0], res["fp64_dict"]["stress"][0]
+    out["mixed_vs_fp64_stress_rel"] = float(np.max(np.abs(a - b)) / np.max(np.abs(b)))
+    out["full_equals_dict"] = bool(np.array_equal(res["mixed_full"]["stress"][0], a))
+    return out
+
+
+def workload_c4(args, mm, configs):
+    """512 RVEs of uneven size, 6 unit load cases each."""
+    specs = configs.c4_specs(count=512)
+    rows = sum(3 * (s.nex + 1) ** 3 for s in specs)
+    budget = int(rows * 250 * 1.3) + (1 << 30)
+    with mm.Engine(nrhs=6, precision=mm.MM_PREC_MIXED, hbm_budget=budget, max_rves=520,
+                   slab_nodes=256, inner_rtol=1e-5, rtol=1e-9) as eng:
+        run_batch(eng, specs[:64], mm)  # warm-up: graph capture
+        t = time.perf_counter()
+        res = run_batch(eng, specs, mm)
+        dt = time.perf_counter() - t
+    its = [r["pcg_its"] for r in res]
+    out = {"workload": "c4: 512 RVEs, n in {8,12,16,20,24,32}, 6 load cases each", "seconds": dt,
+           "tangents_per_s": len(specs) / dt, "dofs": rows, "pcg_its": [min(its), max(its)],
+           "all_converged": all(r["status"] == 0 for r in res)}
+    log("c4: %.3f s -> %.1f tangents/s" % (dt, len(specs) / dt))
+    return out
+
+
+def workload_c5(args, mm, configs):
+    """FE^2 serving stream, two Newton passes over the same gauss points."""
+    count = args.count or 4096
+    rate = args.rate or 600.0
+    stream = configs.c5_stream(count=count, rate=rate)
+    eye = [np.array([1e-3 if a == b else 0.0 for b in range(6)]) for a in range(6)]
+    passes = []
+    with mm.Engine(nrhs=7, precision=mm.MM_PREC_MIXED, hbm_budget=64 << 30, max_rves=count + 64,
+                   slab_nodes=256, inner_rtol=1e-5, rtol=1e-9, steps_per_launch=8) as se:
+        se.start()
+        for ps in range(2):
+            got = []
+            t0 = time.perf_counter()
+            for i, rq in enumerate(stream):
+                while time.perf_counter() - t0 < rq["arrival"]:
+                    got.extend(se.poll())
+                    time.sleep(0.0002)
+                strain = rq["strain"] * (1.0 + 0.05 * ps)  # next Newton iterate
+                se.submit(ps * count + i + 1, rq["spec"], eye + [strain], mm.MM_TANGENT_UNIT,
+                          warm_key=rq["key"], warm_start=ps)
+            while len(got) < count and time.perf_counter() - t0 < 600:
+                got.extend(se.poll())
+                time.sleep(0.0005)
+            ts = time.perf_counter() - t0
+            lat = np.array([(r["t_done"] - r["t_submit"]) * 1e3 for r in got])
+            its = np.array([r["pcg_its"] for r in got])
+            passes.append(dict(requests=len(got), seconds=ts, tangents_per_s=len(got) / ts,
+                               p50_ms=float(np.percentile(lat, 50)), p99_ms=float(np.percentile(lat, 99)),
+                               mean_pcg_its=float(its.mean()),
+                               all_ok=all(r["status"] == 0 for r in got)))
+            log("c5 pass %d: %s" % (ps, passes[-1]))
+        se.stop()
+    return {"workload": "c5: FE^2 stream, %d gauss points (12^3/20^3/30^3 RVEs), Poisson %.0f req/s, "
+                        "strain + 6 unit loads per request; pass 2 warm-started" % (count, rate),
+            "cold": passes[0], "warm": passes[1]}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -72,7 +172,7 @@ def main():
     ap.add_argument("--slab", type=int, default=0, help="nodes per ELL slab (64, 128, 256); 0 = autotune")
     ap.add_argument("--inner-rtol", type=float, default=0.0, help="FP32 PCG target per sweep; 0 = autotune")
     ap.add_argument("--rtol", type=float, default=1e-9, help="FP64 residual target (relative)")
-    ap.add_argument("--tune-batch", type=int, default=16)
+    ap.add_argument("--tune-batch", type=int, default=32)
     ap.add_argument("--spl", type=int, default=16, help="PCG steps per graph launch")
     ap.add_argument("--cpu-sample", type=int, default=2, help="RVE tangents timed on the CPU")
     ap.add_argument("--no-plain", action="store_true")
@@ -80,6 +180,9 @@ def main():
     ap.add_argument("--serve", type=int, default=1024, help="requests in the serving sample")
     ap.add_argument("--no-full-ell", action="store_true", help="skip the full-ELL SpMV figure")
     ap.add_argument("--kernels-only", action="store_true")
+    ap.add_argument("--workload", default="c2", choices=("c2", "c3", "c4", "c5"))
+    ap.add_argument("--rate", type=float, default=0.0, help="c5 arrival rate (requests/s)")
+    ap.add_argument("--count", type=int, default=0, help="c5 requests per pass")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU; multi-GPU runs are not part of it "
@@ -88,6 +191,11 @@ def main():
 
     import madmax_amd as mm
     from madmax_amd import configs
+
+    if args.workload != "c2":
+        line = {"c3": workload_c3, "c4": workload_c4, "c5": workload_c5}[args.workload](args, mm, configs)
+        print(json.dumps(line), flush=True)
+        return
 
     B, n = args.batch, args.n
     specs = [configs.c2_spec(2, i, n=n) for i in range(B)]
@@ -221,7 +329,7 @@ def main():
     # serving sample (FE^2 stream, mixed RVE sizes, Poisson arrivals)
     serving = None
     if args.serve > 0:
-        stream = configs.c5_stream(count=args.serve, rate=4000.0)
+        stream = configs.c5_stream(count=args.serve, rate=600.0)
         se = mm.Engine(nrhs=7, precision=mm.MM_PREC_MIXED, hbm_budget=8 << 30, max_rves=512,
                        slab_nodes=args.slab, inner_rtol=args.inner_rtol, rtol=args.rtol,
                        steps_per_launch=8)

@@ -192,10 +192,15 @@ struct mm_engine {
   std::condition_variable qcv;
   std::deque<Queued> queue;
   std::deque<mm_result> results;
-  // graph
-  hipGraph_t graph = nullptr;
-  hipGraphExec_t gexec = nullptr;
-  int graph_grid = -1;
+  // solver launches run over the slabs of the RVEs being solved: act_list[0]
+  // is their count, act_list[1..] the slabs in ascending order (device), staged
+  // in st_act (pinned); rebuilt when an RVE is admitted or retired
+  int *act_list = nullptr, *st_act = nullptr;
+  bool act_dirty = true;
+  long long act_count = 0;
+  // one captured graph per launch grid (a power of two >= 64 covering the
+  // active slabs); the kernels read the active count from act_list[0]
+  std::map<int, std::pair<hipGraph_t, hipGraphExec_t>> graphs;
   // stats / threading
   std::mutex step_mu;
   long long completed = 0, nsteps = 0;
@@ -227,6 +232,7 @@ struct mm_engine {
     P.bw_tab = bw_tab;
     P.elem_mat = elem_mat;
     P.npat = npat;
+    P.act_list = act_list;
     P.dict = dict;
     P.dict64 = dict64;
     P.SN = SN;
@@ -267,7 +273,8 @@ static int engine_free_arena(mm_engine *e)
                   e->dv_states, e->dv_slots, e->dv_ke, e->dv_slabs, e->dv_phase};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
-  void *hp[] = {e->out_host, e->st_states, e->st_slots, e->st_ke, e->st_slabs, e->st_phase};
+  if (e->act_list) (void)hipFree(e->act_list);
+  void *hp[] = {e->out_host, e->st_states, e->st_slots, e->st_ke, e->st_slabs, e->st_phase, e->st_act};
   for (void *p : hp)
     if (p) (void)hipHostFree(p);
   return 0;
@@ -379,6 +386,7 @@ extern "C" int mm_engine_create(const mm_engine_config *cfg, mm_engine **out)
     else EALLOC(e->dict, (size_t)e->rve_cap * kDictPat * kDictEntry * 4);
   }
   EALLOC(e->npat, (size_t)e->nslab_cap * e->SN);
+  EALLOC(e->act_list, ((size_t)e->nslab_cap + 1) * 4);
   EALLOC(e->dinv, nrows * e->tsz);
   EALLOC(e->x64, nrows * e->K * 8);
   EALLOC(e->d, nrows * e->K * e->tsz);
@@ -401,6 +409,10 @@ extern "C" int mm_engine_create(const mm_engine_config *cfg, mm_engine **out)
     goto fail;
   }
   memset(e->out_host, 0, (size_t)e->rve_cap * sizeof(RveOut));
+  if (hipHostMalloc((void **)&e->st_act, ((size_t)e->nslab_cap + 1) * 4, hipHostMallocDefault) != hipSuccess) {
+    rc = MM_ERR_HIP;
+    goto fail;
+  }
   if (hipHostGetDevicePointer((void **)&e->out_dev, e->out_host, 0) != hipSuccess) {
     rc = MM_ERR_HIP;
     goto fail;
@@ -432,8 +444,10 @@ extern "C" int mm_engine_destroy(mm_engine *e)
   if (!e) return MM_ERR_ARG;
   mm_engine_stop(e);
   if (e->stream) (void)hipStreamSynchronize(e->stream);
-  if (e->gexec) (void)hipGraphExecDestroy(e->gexec);
-  if (e->graph) (void)hipGraphDestroy(e->graph);
+  for (auto &g : e->graphs) {
+    (void)hipGraphExecDestroy(g.second.second);
+    (void)hipGraphDestroy(g.second.first);
+  }
   engine_free_arena(e);
   if (e->stream) (void)hipStreamDestroy(e->stream);
   delete e;
@@ -668,12 +682,14 @@ static int admit(mm_engine *e)
   if ((rc = eng_launch_assemble(e->fp64, e->K, P, (int)nsl_re, e->dv_slabs + nsl, ASM_X_ONLY, e->stream)))
     return rc;
   e->n_active += (int)n;
+  e->act_dirty = true;
   return 0;
 }
 
-// Slabs launched: the allocated ones rounded up to a multiple of 64 (whole
-// waves of workgroups per XCD), never beyond the arena, and always a multiple
-// of 8 so the XCD-aware slab order is a permutation of the grid.
+// Slabs launched by the stand-alone timing kernels: every allocated slab,
+// rounded up to a multiple of 64 (whole waves of workgroups per XCD), never
+// beyond the arena, and always a multiple of 8 so the XCD-aware slab order is
+// a permutation of the grid.
 static int solver_grid(const mm_engine *e, long long hw)
 {
   long long g = ((hw + 63) / 64) * 64;
@@ -681,33 +697,55 @@ static int solver_grid(const mm_engine *e, long long hw)
   return (int)g;
 }
 
+// Rebuilds the list of slabs being solved (ascending, so each XCD walks
+// contiguous slabs) and queues its upload on the engine stream.  The pinned
+// staging buffer is free: the previous step ended with a stream sync.
+static int update_active(mm_engine *e)
+{
+  if (!e->act_dirty) return 0;
+  std::vector<std::pair<long long, long long>> runs;
+  for (int s = 0; s < e->rve_cap; s++)
+    if (e->slots[s].state == SL_ACTIVE) runs.push_back({e->slots[s].first_slab, e->slots[s].nslab});
+  std::sort(runs.begin(), runs.end());
+  long long n = 0;
+  for (auto &r : runs)
+    for (long long b = 0; b < r.second; b++) e->st_act[1 + n++] = (int)(r.first + b);
+  e->st_act[0] = (int)n;
+  MM_HIP(hipMemcpyAsync(e->act_list, e->st_act, (size_t)(n + 1) * 4, hipMemcpyHostToDevice, e->stream));
+  e->act_count = n;
+  e->act_dirty = false;
+  return 0;
+}
+
 static int run_steps(mm_engine *e)
 {
-  long long hw = e->slabs.high_water();
-  if (hw <= 0) return 0;
-  const int grid = solver_grid(e, hw);
+  int rc = update_active(e);
+  if (rc) return rc;
+  if (e->act_count <= 0) return 0;
+  int grid = 64;
+  while (grid < e->act_count) grid *= 2;
   EngPtrs P = e->ptrs();
   SolveParams sp = e->params();
   const int T = e->cfg.steps_per_launch;
   if (e->cfg.use_graph) {
-    if (!e->gexec || e->graph_grid != grid) {
-      if (e->gexec) (void)hipGraphExecDestroy(e->gexec);
-      if (e->graph) (void)hipGraphDestroy(e->graph);
-      e->gexec = nullptr;
-      e->graph = nullptr;
+    auto it = e->graphs.find(grid);
+    if (it == e->graphs.end()) {
       MM_HIP(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
-      int rc = eng_launch_steps(e->fp64, e->K, e->dict_mode, P, sp, grid, T, e->stream);
+      rc = eng_launch_steps(e->fp64, e->K, e->dict_mode, P, sp, grid, T, e->stream);
       hipGraph_t g = nullptr;
       hipError_t ce = hipStreamEndCapture(e->stream, &g);
       if (rc) return rc;
       MM_HIP(ce);
-      e->graph = g;
-      MM_HIP(hipGraphInstantiate(&e->gexec, g, nullptr, nullptr, 0));
-      e->graph_grid = grid;
+      hipGraphExec_t x = nullptr;
+      if (hipGraphInstantiate(&x, g, nullptr, nullptr, 0) != hipSuccess) {
+        (void)hipGraphDestroy(g);
+        return MM_ERR_HIP;
+      }
+      it = e->graphs.emplace(grid, std::make_pair(g, x)).first;
     }
-    MM_HIP(hipGraphLaunch(e->gexec, e->stream));
+    MM_HIP(hipGraphLaunch(it->second.second, e->stream));
   } else {
-    int rc = eng_launch_steps(e->fp64, e->K, e->dict_mode, P, sp, grid, T, e->stream);
+    rc = eng_launch_steps(e->fp64, e->K, e->dict_mode, P, sp, grid, T, e->stream);
     if (rc) return rc;
   }
   return 0;
@@ -738,6 +776,7 @@ static void retire(mm_engine *e)
     res.t_done = e->now();
     o->phase = 0;
     e->n_active--;
+    e->act_dirty = true;
     e->completed++;
     if (h.req.warm_key) {
       h.state = SL_CACHED;

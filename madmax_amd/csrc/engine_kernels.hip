@@ -62,6 +62,22 @@ __device__ __forceinline__ int xcd_slab()
   return (blockIdx.x % kXcd) * per + blockIdx.x / kXcd;
 }
 
+// Solver launches run over the list of slabs being solved (act_list[0] of
+// them, ascending from act_list[1]); the grid is a power of two >= 64 that
+// covers the list.  The list is split in 8 contiguous parts, one per XCD, so
+// neighbouring slabs of an RVE share an L2.  Returns false for workgroups
+// beyond the list.
+__device__ __forceinline__ bool active_slab(const EngPtrs &P, int &slab)
+{
+  const int n = P.act_list[0];
+  const int per = (n + kXcd - 1) / kXcd;
+  const int j = blockIdx.x / kXcd;
+  const int i = (blockIdx.x % kXcd) * per + j;
+  if (j >= per || i >= n) return false;
+  slab = P.act_list[1 + i];
+  return true;
+}
+
 __device__ __forceinline__ int node_class(int ix, int iy, int iz, int nx, int ny, int nz)
 {
   const int cx = (ix == 0) ? 0 : ((ix == nx - 1) ? 2 : 1);
@@ -778,7 +794,9 @@ __global__ __launch_bounds__(256) void k_step_a(EngPtrs P, SolveParams sp)
   __shared__ double lds[RedLds<K>::size];
   __shared__ SolveState N;
   __shared__ int s_xupd;
-  const int slab = xcd_slab(), tid = threadIdx.x, SN = P.SN, RS = 3 * SN;
+  int slab;
+  if (!active_slab(P, slab)) return;
+  const int tid = threadIdx.x, SN = P.SN, RS = 3 * SN;
   const int slot = P.slab_rve[slab];
   if (slot < 0) return;
   RveDev *R = P.rves + slot;
@@ -840,7 +858,9 @@ __global__ __launch_bounds__(256) void k_step_b(EngPtrs P, SolveParams sp)
   using T2 = typename Pair<TS>::type;
   __shared__ double lds[2 * K * 16];
   __shared__ double s_alpha[K];
-  const int slab = xcd_slab(), tid = threadIdx.x, RS = 3 * P.SN;
+  int slab;
+  if (!active_slab(P, slab)) return;
+  const int tid = threadIdx.x, RS = 3 * P.SN;
   const int slot = P.slab_rve[slab];
   if (slot < 0) return;
   RveDev *R = P.rves + slot;
@@ -1064,7 +1084,9 @@ template <typename TS, int K>
 __global__ __launch_bounds__(256) void k_step_r(EngPtrs P, SolveParams sp)
 {
   __shared__ double lds[RedLds<K>::size];
-  const int slab = xcd_slab(), tid = threadIdx.x;
+  int slab;
+  if (!active_slab(P, slab)) return;
+  const int tid = threadIdx.x;
   const int slot = P.slab_rve[slab];
   if (slot < 0) return;
   RveDev *R = P.rves + slot;

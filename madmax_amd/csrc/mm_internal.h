@@ -144,6 +144,7 @@ struct EngPtrs {
   const double *ke_tab, *c_tab, *bw_tab;
   const uint8_t *elem_mat;
   const uint8_t *npat;  // [slab][SN] material pattern of the node's 8 elements
+  const int *act_list;  // [0]: number of slabs being solved, [1..]: those slabs
   const void *dict;     // [rve slot][256][27][12] interior 3-row ELL blocks (TS)
   const double *dict64; // the same in FP64 (the residual's matrix)
   int SN;        // nodes per slab = lanes per workgroup (a power of two)
