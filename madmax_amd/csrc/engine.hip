@@ -974,7 +974,7 @@ static int download_ell_dict(mm_engine *e, int s, int *cols, double *vals)
           if (cls != 13) {
             v = (m == n && dd == d) ? 1.0 : 0.0;
           } else {
-            const size_t src = (size_t)pat[n] * kDictEntry + (size_t)t * kDictBlk + dd * 4 + d;
+            const size_t src = (size_t)pat[n] * kDictEntry + (size_t)key * kDictBlk + dd * 4 + d;
             v = e->fp64 ? ((const double *)dict.data())[src] : (double)((const float *)dict.data())[src];
           }
         }
@@ -1003,17 +1003,32 @@ extern "C" int mm_engine_download_ell(mm_engine *e, uint64_t key, int *cols, dou
   MM_HIP(hipMemcpyAsync(hv.data(), (char *)e->vals + (size_t)h.first_slab * RS * kW * e->tsz,
                         nv * e->tsz, hipMemcpyDeviceToHost, e->stream));
   MM_HIP(hipStreamSynchronize(e->stream));
-  // the reference's column of slot 3t + d' of row 3n + d is 3 * bcols[n][t] + d'
-  for (int row = 0; row < h.nrow; row++) {
-    const int n = row / 3, d = row % 3;
+  // block key of a node holds the reference's slot t = key_slot[class][key]:
+  // column 3 * bcols[n][key] + d' sits at slot 3t + d' of row 3n + d
+  SlotTables T;
+  build_slot_tables(&T);
+  const mm_rve_spec &rs = h.req.rve;
+  const int nx = rs.nex + 1, ny = rs.ney + 1, nz = rs.nez + 1;
+  auto cls_of = [](int i, int n) { return i == 0 ? 0 : (i == n - 1 ? 2 : 1); };
+  for (size_t i = 0; i < (size_t)h.nrow * kW; i++) {
+    cols[i] = -1;
+    vals[i] = 0.0;
+  }
+  for (int n = 0; n < h.nrow / 3; n++) {
+    const int ix = n % nx, iy = (n / nx) % ny, iz = n / (nx * ny);
+    const int cls = cls_of(ix, nx) + 3 * cls_of(iy, ny) + 9 * cls_of(iz, nz);
     const size_t sl = (size_t)(n / e->SN), lane = n % e->SN;
-    for (int j = 0; j < kW; j++) {
-      const int t = j / 3, dd = j % 3;
-      const int m = hc[sl * e->SN * 27 + (size_t)t * e->SN + lane];
-      cols[(size_t)row * kW + j] = (m < 0) ? -1 : 3 * m + dd;
-      const size_t src = sl * RS * kW + (size_t)j * RS + 3 * lane + d;
-      vals[(size_t)row * kW + j] =
-          e->fp64 ? ((const double *)hv.data())[src] : (double)((const float *)hv.data())[src];
+    for (int key = 0; key < 27; key++) {
+      const int t = T.key_slot[cls][key];
+      if (t < 0) continue;
+      const int m = hc[sl * e->SN * 27 + (size_t)key * e->SN + lane];
+      for (int d = 0; d < 3; d++)
+        for (int dd = 0; dd < 3; dd++) {
+          const size_t dst = (size_t)(3 * n + d) * kW + 3 * t + dd;
+          const size_t src = sl * RS * kW + (size_t)(3 * key + dd) * RS + 3 * lane + d;
+          cols[dst] = (m < 0) ? -1 : 3 * m + dd;
+          vals[dst] = e->fp64 ? ((const double *)hv.data())[src] : (double)((const float *)hv.data())[src];
+        }
     }
   }
   return 0;

@@ -2,17 +2,22 @@
 //
 // Arena layout in HBM.  Rows (dofs) are numbered 3*node + d; a slab holds SN
 // consecutive nodes (3*SN rows) of one RVE and is processed by one workgroup
-// of SN lanes, one lane per node:
-//   vals  : [slab][81][3*SN]  ELL values in the reference's slot order
-//                             (slot j = 3t + d' of row 3n + d), column-major
-//                             inside the slab so lane i reads the three rows
-//                             of its node as 12 contiguous bytes
-//   bcols : [slab][27][SN]    RVE-local node id of slot block t (-1 = empty);
-//                             the reference's column of slot 3t + d' is
-//                             3*bcols + d', shared by the node's three rows
+// of SN lanes, one lane per node.  A node's 27 neighbour blocks are kept in
+// neighbour-key order, key = (dx+1) + 3(dy+1) + 9(dz+1); g_slots.key_slot
+// maps a key to the reference's slot of the block in the node's rows.
+// Full storage:
+//   vals  : [slab][81][3*SN]  value of column 3m + d' of row 3n + d at block
+//                             position 3*key + d', column-major inside the
+//                             slab so lane i reads its three rows as 12
+//                             contiguous bytes
+//   bcols : [slab][27][SN]    RVE-local neighbour node of each key (-1 = none)
+// Dictionary storage:
+//   dict  : [rve][256][27][12] the interior blocks per phase pattern
+//   npat  : [slab][SN]        the node's pattern
+// Both:
 //   vectors [slab][K][3*SN]   x64 (FP64 displacement), d (correction), r, z,
 //                             p, q (PCG); K = load cases
-//   dinv  : [row]             Jacobi preconditioner
+//   dinv  : [slab][3*SN]      Jacobi preconditioner
 //   part  : [slab][96]        per-workgroup partial sums (FP64)
 // Every resident RVE owns a contiguous run of slabs; its first row is
 // first_slab * 3 * SN.
@@ -293,13 +298,12 @@ __global__ void k_assemble(int nblk, const int *__restrict__ slab_list, const in
     for (int d = 0; d < 3; d++) dinv[grow0 + d] = (TV)(1.0 / diag[d]);
   }
   if (mode == ASM_FULL) {
-    const int ns = g_slots.nslot[cls];
-    const int dslot = g_slots.diag_slot[cls];
     const double *ke0 = ke_tab + (size_t)slot * kNmat * kKe;
     double diag[3] = {0.0, 0.0, 0.0};
-    for (int t = 0; t < 27; t++) {
-      if (t < ns) {
-        const int key = g_slots.off[cls][t];
+    // blocks stored in neighbour-key order; slot t is the reference's position
+    for (int key = 0; key < 27; key++) {
+      const int t = g_slots.key_slot[cls][key];
+      if (t >= 0) {
         const int m = n + (key % 3 - 1) + ((key / 3) % 3 - 1) * nx + (key / 9 - 1) * nxy;
         double v[3][3];  // [d][d']
         if (bc) {
@@ -318,15 +322,16 @@ __global__ void k_assemble(int nblk, const int *__restrict__ slab_list, const in
                 v[d][dd] = (c == 0) ? ke[d * kNd + dd] : v[d][dd] + ke[d * kNd + dd];
           }
         }
-        if (t == dslot)
+        if (key == 13)
           for (int d = 0; d < 3; d++) diag[d] = v[d][d];
-        bcols[cb + (long long)t * SN + tid] = m;
+        bcols[cb + (long long)key * SN + tid] = m;
         for (int dd = 0; dd < 3; dd++)
-          for (int d = 0; d < 3; d++) vals[vb + (long long)(3 * t + dd) * RS + 3 * tid + d] = (TV)v[d][dd];
+          for (int d = 0; d < 3; d++)
+            vals[vb + (long long)(3 * key + dd) * RS + 3 * tid + d] = (TV)v[d][dd];
       } else {
-        bcols[cb + (long long)t * SN + tid] = -1;
+        bcols[cb + (long long)key * SN + tid] = -1;
         for (int dd = 0; dd < 3; dd++)
-          for (int d = 0; d < 3; d++) vals[vb + (long long)(3 * t + dd) * RS + 3 * tid + d] = (TV)0;
+          for (int d = 0; d < 3; d++) vals[vb + (long long)(3 * key + dd) * RS + 3 * tid + d] = (TV)0;
       }
     }
     for (int d = 0; d < 3; d++) dinv[grow0 + d] = (TV)(1.0 / diag[d]);
@@ -364,7 +369,8 @@ __global__ __launch_bounds__(384) void k_dict(int n, const int *__restrict__ slo
   const int tid = threadIdx.x;
   if (tid >= kDictEntry) return;
   const int slot = slots[i];
-  const int t = tid / kDictBlk, dd = (tid % kDictBlk) / 4, d = tid % 4;
+  const int key = tid / kDictBlk, dd = (tid % kDictBlk) / 4, d = tid % 4;
+  const int t = g_slots.key_slot[13][key];
   double v = 0.0;
   if (d < 3) {
     const double *ke0 = ke_tab + (size_t)slot * kNmat * kKe;
@@ -450,9 +456,9 @@ __device__ __forceinline__ void scatter_node(T *__restrict__ x, long long nb, in
     for (int d = 0; d < 3; d++) x[nb + (long long)k * RS + d] = v[d][k];
 }
 
-// Row block of lane tid (node n) times z, full storage: the 27 slot blocks
-// visited in the reference's slot order, w[d] accumulated over slots
-// j = 3t + d'.  zc receives the node's own z rows.
+// Row block of lane tid (node n) times z, full storage: the 27 neighbour
+// blocks in key order, w[d] accumulated over keys and d'.  zc receives the
+// node's own z rows.
 template <typename TS, int K>
 __device__ __forceinline__ void node_spmv(const EngPtrs &P, int first_slab, int slab, int tid, int n,
                                           const TS *__restrict__ z, TS (&w)[3][K], TS (&zc)[3][K])
@@ -466,14 +472,14 @@ __device__ __forceinline__ void node_spmv(const EngPtrs &P, int first_slab, int 
 #pragma unroll
     for (int k = 0; k < K; k++) w[d][k] = (TS)0;
 #pragma unroll 3
-  for (int t = 0; t < 27; t++) {
-    const int m = __builtin_nontemporal_load(&bcols[cb + (long long)t * SN + tid]);
+  for (int key = 0; key < 27; key++) {
+    const int m = __builtin_nontemporal_load(&bcols[cb + (long long)key * SN + tid]);
     TS v[3][3];  // [d'][d]
 #pragma unroll
     for (int dd = 0; dd < 3; dd++)
 #pragma unroll
       for (int d = 0; d < 3; d++)
-        v[dd][d] = __builtin_nontemporal_load(&vals[vb + (long long)(3 * t + dd) * RS + 3 * tid + d]);
+        v[dd][d] = __builtin_nontemporal_load(&vals[vb + (long long)(3 * key + dd) * RS + 3 * tid + d]);
     if (m >= 0) {
       const long long nb = node_vec(P, first_slab, m, K);
 #pragma unroll
@@ -498,26 +504,26 @@ __device__ __forceinline__ T uniform_load(const T *p)
   return *(const __attribute__((address_space(4))) T *)p;
 }
 
-// The 27-block product of an interior node from dictionary entry D; UNI: D is
-// the same in every lane and its values are read through the scalar unit.
+// The 27-block product of an interior node from dictionary entry D, the
+// neighbours gathered in key order.  UNI: D is the same in every lane of the
+// wavefront and its values are read through the scalar unit.
 template <typename TS, int K, bool UNI>
 __device__ __forceinline__ void dict_rows(const EngPtrs &P, const TS *D, int first_slab, int n, int nx, int nxy,
                                           const TS *__restrict__ z, TS (&w)[3][K], TS (&zc)[3][K])
 {
   const int RS = 3 * P.SN;
 #pragma unroll 3
-  for (int t = 0; t < 27; t++) {
-    const int key = g_slots.off[13][t];
+  for (int key = 0; key < 27; key++) {
     const int m = n + (key % 3 - 1) + ((key / 3) % 3 - 1) * nx + (key / 9 - 1) * nxy;
     TS v[3][4];  // [d'][d]
     if (UNI) {
 #pragma unroll
       for (int dd = 0; dd < 3; dd++)
 #pragma unroll
-        for (int d = 0; d < 3; d++) v[dd][d] = uniform_load(D + t * kDictBlk + dd * 4 + d);
+        for (int d = 0; d < 3; d++) v[dd][d] = uniform_load(D + key * kDictBlk + dd * 4 + d);
     } else {
 #pragma unroll
-      for (int dd = 0; dd < 3; dd++) load_vec<TS, 4>(D + t * kDictBlk + dd * 4, v[dd]);
+      for (int dd = 0; dd < 3; dd++) load_vec<TS, 4>(D + key * kDictBlk + dd * 4, v[dd]);
     }
     const long long nb = node_vec(P, first_slab, m, K);
 #pragma unroll
@@ -539,7 +545,7 @@ __device__ __forceinline__ void dict_rows(const EngPtrs &P, const TS *D, int fir
 // The same product from the dictionary: an interior node's 27 blocks come
 // from entry npat of its RVE at the structured neighbour offsets; a boundary
 // node's row is the identity, whose full-ELL product z + 0 is reproduced
-// exactly.  Same fma order as node_spmv, so the same bits.  When every
+// exactly.  Same fma order as node_spmv (key order), so the same bits.  When every
 // interior lane of the wavefront has the same pattern (the common case) the
 // block values are read once per wavefront through the scalar unit instead of
 // once per lane through L1.  zc receives the node's own z rows (the centre
@@ -585,6 +591,10 @@ __device__ __forceinline__ void slab_node_spmv(const EngPtrs &P, const RveDev *R
   else node_spmv<TS, K>(P, R->first_slab, slab, tid, n, (const TS *)P.z, w, zc);
 }
 
+// Dot products accumulate with explicit fma, so their rounding is the same in
+// every code path that computes them (the two matrix storages give the same
+// bits only if it is).
+//
 // Element-wise passes: lane tid handles the value pairs tid, tid + blockDim,
 // ... of each load case's run of `rows` valid rows (a pair is 8 contiguous
 // bytes in FP32), the odd last row by lane 0.
@@ -768,7 +778,7 @@ __device__ __forceinline__ void inner_a(const EngPtrs &P, const RveDev *R, int s
         const TS qi = fma(b, q[i + d], w[d][k]);
         p[i + d] = pi;
         q[i + d] = qi;
-        v[k] += (double)pi * (double)qi;
+        v[k] = fma((double)pi, (double)qi, v[k]);
       }
     }
   }
@@ -907,10 +917,10 @@ __global__ __launch_bounds__(256) void k_step_b(EngPtrs P, SolveParams sp)
       *reinterpret_cast<T2 *>(dv + i) = di;
       *reinterpret_cast<T2 *>(rv + i) = ri;
       *reinterpret_cast<T2 *>(zv + i) = zi;
-      vu[k] += (double)ri.x * (double)ri.x;
-      vu[k] += (double)ri.y * (double)ri.y;
-      vu[K + k] += (double)ri.x * (double)zi.x;
-      vu[K + k] += (double)ri.y * (double)zi.y;
+      vu[k] = fma((double)ri.x, (double)ri.x, vu[k]);
+      vu[k] = fma((double)ri.y, (double)ri.y, vu[k]);
+      vu[K + k] = fma((double)ri.x, (double)zi.x, vu[K + k]);
+      vu[K + k] = fma((double)ri.y, (double)zi.y, vu[K + k]);
     }
     if ((rows & 1) && tid == 0) {
       const long long i = kb + rows - 1;
@@ -920,8 +930,8 @@ __global__ __launch_bounds__(256) void k_step_b(EngPtrs P, SolveParams sp)
       dv[i] = di;
       rv[i] = ri;
       zv[i] = zi;
-      vu[k] += (double)ri * (double)ri;
-      vu[K + k] += (double)ri * (double)zi;
+      vu[k] = fma((double)ri, (double)ri, vu[k]);
+      vu[K + k] = fma((double)ri, (double)zi, vu[K + k]);
     }
   }
   block_sum<2 * K>(vu, lds);
@@ -971,15 +981,14 @@ __device__ void res_slab(const EngPtrs &P, int slab, int slot, const RveDev *R, 
       const double *D = P.dict64 + ((size_t)slot * kDictPat + P.npat[(long long)slab * P.SN + tid]) *
                                        kDictEntry;
 #pragma unroll 1
-      for (int t = 0; t < 27; t++) {
-        const int key = g_slots.off[13][t];
+      for (int key = 0; key < 27; key++) {
         const int dx = key % 3 - 1, dy = (key / 3) % 3 - 1, dz = key / 9 - 1;
         const int m = n + dx + dy * nx + dz * nxy;
         const int jx = ix + dx, jy = iy + dy, jz = iz + dz;
         const bool nb_bc = jx == 0 || jx == nx - 1 || jy == 0 || jy == ny - 1 || jz == 0 || jz == nz - 1;
         double vb[3][4];
 #pragma unroll
-        for (int dd = 0; dd < 3; dd++) load_vec<double, 4>(D + t * kDictBlk + dd * 4, vb[dd]);
+        for (int dd = 0; dd < 3; dd++) load_vec<double, 4>(D + key * kDictBlk + dd * 4, vb[dd]);
         double xv[3][K];
         gather_node<double, K>(P.x64, node_vec(P, R->first_slab, m, K), RS, xv);
 #pragma unroll
@@ -1009,10 +1018,10 @@ __device__ void res_slab(const EngPtrs &P, int slab, int slot, const RveDev *R, 
         ri[d][k] = (TS)rk;
         zi[d][k] = dg * ri[d][k];
         zero[d][k] = (TS)0;
-        v[k] += rk * rk;
-        v[K + k] += (double)ri[d][k] * (double)ri[d][k];
-        v[2 * K + k] += (double)ri[d][k] * (double)zi[d][k];
-        v[3 * K + k] += rc64[d][k] * rc64[d][k];
+        v[k] = fma(rk, rk, v[k]);
+        v[K + k] = fma((double)ri[d][k], (double)ri[d][k], v[K + k]);
+        v[2 * K + k] = fma((double)ri[d][k], (double)zi[d][k], v[2 * K + k]);
+        v[3 * K + k] = fma(rc64[d][k], rc64[d][k], v[3 * K + k]);
       }
     }
     scatter_node<TS, K>((TS *)P.r, nb, RS, ri);

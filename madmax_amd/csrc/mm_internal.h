@@ -43,6 +43,7 @@ struct SlotTables {
   uint8_t diag_slot[27];         // slot of the node itself
   uint8_t nelem[27];             // elements around the node (1..8)
   uint8_t elem[27][8];           // (ox+1) | (oy+1)<<1 | (oz+1)<<2 | a<<3, ascending id
+  int8_t key_slot[27][27];       // slot of neighbour key (dx+1)+3(dy+1)+9(dz+1), -1 if none
 };
 
 void build_slot_tables(SlotTables *t);
@@ -151,9 +152,14 @@ struct EngPtrs {
   int sn_shift;  // log2(SN)
 };
 
-// Dictionary ELL: entry (slot, pattern) holds, for the 27 neighbour blocks t
-// of an interior node in the reference's slot order, v[d'][d] (value of row
-// 3n+d, column 3m+d') at [t * 12 + d' * 4 + d] (padded to 16-byte rows).
+// Both matrix storages keep a node's 27 neighbour blocks in neighbour-key
+// order, key = (dx+1) + 3(dy+1) + 9(dz+1) (z-major, x fastest), so the SpMV
+// visits the three x-neighbours of a (dy, dz) strip one after the other.
+// key_slot maps a key to the reference's slot of that block in the node's
+// row (first-touch order), which fixes the values and the ELL download.
+// Dictionary ELL: entry (slot, pattern) holds, for the 27 neighbour keys of
+// an interior node, v[d'][d] (value of row 3n+d, column 3m+d') at
+// [key * 12 + d' * 4 + d] (padded to 16-byte rows).
 constexpr int kDictPat = 256;
 constexpr int kDictBlk = 12;
 constexpr int kDictEntry = 27 * kDictBlk;

@@ -50,6 +50,7 @@ void build_slot_tables(SlotTables *t)
                 t->contrib[cls][s][t->ncontrib[cls][s]++] = (uint16_t)(eo | (a << 3) | (b << 6));
               }
             }
+        for (int key = 0; key < 27; key++) t->key_slot[cls][key] = (int8_t)slot_of[key];
         t->nslot[cls] = nslot;
         t->nelem[cls] = (uint8_t)nelem;
         t->diag_slot[cls] = (uint8_t)slot_of[13];
