@@ -281,12 +281,14 @@ def main():
     copy_gbs = 2 * copy_bytes / (copy_ms.value * 1e-3) / 1e9
     log("device copy: %.0f GB/s (read + write of 4 GiB)" % copy_gbs)
 
-    # plain FP64 single-RVE GPU path: one load case at a time, host check per iteration
+    # plain FP64 single-RVE GPU path: the reference's algorithm (zero interior
+    # start, one load case at a time), host check per iteration
     plain = None
     if not args.no_plain:
         pe = mm.Engine(nrhs=1, precision=mm.MM_PREC_FP64, hbm_budget=per_rve * 2 + (64 << 20),
                        max_rves=2, slab_nodes=128, inner_rtol=1e-10, rtol=0.0, abs_tol=1e-4,
-                       max_sweeps=2, sync_every_step=True, use_graph=False)
+                       max_sweeps=2, sync_every_step=True, use_graph=False,
+                       initial_guess=mm.MM_GUESS_ZERO)
         nplain = 2
         t = time.perf_counter()
         cs = []

@@ -140,6 +140,15 @@ int micro_get_displacement(double *x);
 #define MM_ELL_FULL 0
 #define MM_ELL_DICT 1
 
+/* Interior displacement a cold solve starts from.  ZERO is the reference's
+ * start (set_disp_0 sets the boundary only); AFFINE starts every node at the
+ * applied affine field u = E X, which is exact for a homogeneous cell and
+ * leaves only the residual of the phase interfaces (fewer PCG iterations, the
+ * same converged solution; tolerances stay relative to the zero-start
+ * right-hand side). */
+#define MM_GUESS_ZERO 0
+#define MM_GUESS_AFFINE 1
+
 #define MM_TANGENT_NONE 0
 #define MM_TANGENT_UNIT 1 /* c[j*6+i] = sigma_i[j] / strain_i[i], loads i = 0..5 */
 #define MM_TANGENT_FD 2   /* c[j*6+i] = (sigma_{i+1}[j]-sigma_0[j]) / (eps_{i+1}[i]-eps_0[i]) */
@@ -161,6 +170,7 @@ typedef struct {
   int use_graph;         /* capture the steps in a hipGraph */
   int sync_every_step;   /* plain path: host reads the state after every step */
   int ell_storage;       /* MM_ELL_FULL or MM_ELL_DICT (default DICT) */
+  int initial_guess;     /* MM_GUESS_ZERO or MM_GUESS_AFFINE (default AFFINE) */
 } mm_engine_config;
 
 typedef struct {

@@ -20,6 +20,7 @@ MM_NOT_CONVERGED, MM_ERR_HIP, MM_ERR_STATE, MM_REJECTED = 5, 6, 7, 8
 MICRO_SOLVER_PCG, MICRO_SOLVER_PCG_MIXED = 0, 1
 MM_PREC_FP64, MM_PREC_MIXED = 0, 1
 MM_ELL_FULL, MM_ELL_DICT = 0, 1
+MM_GUESS_ZERO, MM_GUESS_AFFINE = 0, 1
 MM_TANGENT_NONE, MM_TANGENT_UNIT, MM_TANGENT_FD = 0, 1, 2
 
 dp = C.POINTER(C.c_double)
@@ -75,7 +76,8 @@ class EngineConfig(C.Structure):
                 ("rtol", C.c_double), ("abs_tol", C.c_double), ("inner_rtol", C.c_double),
                 ("max_inner_its", C.c_int), ("max_sweeps", C.c_int),
                 ("steps_per_launch", C.c_int), ("use_graph", C.c_int),
-                ("sync_every_step", C.c_int), ("ell_storage", C.c_int)]
+                ("sync_every_step", C.c_int), ("ell_storage", C.c_int),
+                ("initial_guess", C.c_int)]
 
 
 class Request(C.Structure):

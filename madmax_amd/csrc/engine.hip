@@ -299,6 +299,7 @@ extern "C" int mm_engine_config_default(mm_engine_config *c)
   c->use_graph = 1;
   c->sync_every_step = 0;
   c->ell_storage = MM_ELL_DICT;
+  c->initial_guess = MM_GUESS_AFFINE;
   return 0;
 }
 
@@ -313,6 +314,7 @@ extern "C" int mm_engine_create(const mm_engine_config *cfg, mm_engine **out)
     return MM_ERR_ARG;
   if (cfg->precision != MM_PREC_FP64 && cfg->precision != MM_PREC_MIXED) return MM_ERR_ARG;
   if (cfg->ell_storage != MM_ELL_FULL && cfg->ell_storage != MM_ELL_DICT) return MM_ERR_ARG;
+  if (cfg->initial_guess != MM_GUESS_ZERO && cfg->initial_guess != MM_GUESS_AFFINE) return MM_ERR_ARG;
   if (!(cfg->rtol >= 0) || !(cfg->inner_rtol > 0) || cfg->max_inner_its < 1 || cfg->max_sweeps < 1)
     return MM_ERR_ARG;
   int ndev = 0;
@@ -547,6 +549,7 @@ static void fill_state(const mm_engine *e, const HostSlot &h, RveDev *st, int wa
   st->tangent_mode = h.req.tangent_mode;
   st->nload = h.req.nload;
   st->warm = warm;
+  st->affine = (e->cfg.initial_guess == MM_GUESS_AFFINE) ? 1 : 0;
   for (int k = 0; k < h.req.nload; k++)
     for (int i = 0; i < MM_NVOI; i++) st->load[k][i] = h.req.strain[k][i];
   for (int c = 0; c < 3; c++) st->st[c].phase = PH_RES;

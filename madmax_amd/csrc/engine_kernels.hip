@@ -336,11 +336,12 @@ __global__ void k_assemble(int nblk, const int *__restrict__ slab_list, const in
     }
     for (int d = 0; d < 3; d++) dinv[grow0 + d] = (TV)(1.0 / diag[d]);
   }
-  // initial displacement for every load case
+  // initial displacement for every load case: affine on the boundary; inside
+  // zero, the affine field, or (warm start) the kept values
   const double X[3] = {ix * R.hx, iy * R.hy, iz * R.hz};
   for (int d = 0; d < 3; d++)
     for (int k = 0; k < K; k++) {
-      if (k < R.nload && bc) {
+      if (k < R.nload && (bc || (R.affine && !R.warm))) {
         const double *s = R.load[k];
         double u;
         if (d == 0) u = s[0] * X[0] + s[3] / 2 * X[1] + s[5] / 2 * X[2];
@@ -628,7 +629,7 @@ template <int K>
 __device__ void start_sweep(SolveState &N, const double (&s4)[4 * K], const RveDev *R,
                             const SolveParams &sp)
 {
-  const bool need_cold = R->warm && N.sweep == 0;
+  const bool need_cold = (R->warm || R->affine) && N.sweep == 0;
   unsigned done_out = 0;
   for (int k = 0; k < K; k++) {
     const double rn = sqrt(s4[k]);
@@ -949,8 +950,8 @@ __global__ __launch_bounds__(256) void k_step_b(EngPtrs P, SolveParams sp)
 // and the start of the next PCG sweep (d = 0, z = dinv r, p = q = 0): every
 // refinement sweep solves the correction equation from zero.  The relative
 // target refers to the cold-start right-hand side (interior displacement
-// zero, the ||b|| of the first solve); a warm-started RVE measures it at
-// sweep 0 from the boundary neighbours alone.
+// zero, the ||b|| of the first solve); an RVE started from the affine field
+// or warm-started measures it at sweep 0 from the boundary neighbours alone.
 // PH_AVG: per-slab partial volume averages of strain and stress in FP64,
 // sum_gp (.) w_gp per element through sum_gp B w, one load case at a time.
 template <typename TS, int K>
@@ -1107,7 +1108,7 @@ __global__ __launch_bounds__(256) void k_step_r(EngPtrs P, SolveParams sp)
     else if (phase == PH_AVG) N.phase = PH_AVGRED;
     R->st[2] = N;
   }
-  if (phase == PH_RES) res_slab<TS, K>(P, slab, slot, R, R->warm && S->sweep == 0, lds);
+  if (phase == PH_RES) res_slab<TS, K>(P, slab, slot, R, (R->warm || R->affine) && S->sweep == 0, lds);
   else if (phase == PH_AVG) avg_slab<TS, K>(P, slab, slot, R, lds);
 }
 

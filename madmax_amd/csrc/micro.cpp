@@ -103,8 +103,10 @@ extern "C" int micro_init(const micro_config *cfg)
   ec.use_graph = 1;
   ec.steps_per_launch = 32;
   if (cfg->solver == MICRO_SOLVER_PCG) {
-    // Newton loop of homog_fe2 with a FP64 PCG solve per iteration
+    // Newton loop of homog_fe2 with a FP64 PCG solve per iteration, from the
+    // reference's start (boundary values, interior zero)
     ec.precision = MM_PREC_FP64;
+    ec.initial_guess = MM_GUESS_ZERO;
     ec.rtol = 0.0;
     ec.abs_tol = cfg->nl_min_norm;
     ec.inner_rtol = cfg->rtol;

@@ -109,6 +109,8 @@ struct RveDev {
   long long elem_base;  // offset into the element-material arena
   double hx, hy, hz, vol;
   int nfib, tangent_mode, nload, warm;
+  int affine;  // cold start from the affine field u = E X (else interior zero)
+  int pad_a;
   double fx[MM_MAX_FIBERS], fy[MM_MAX_FIBERS], fr[MM_MAX_FIBERS];
   double young[kNmat], poisson[kNmat];
   double load[MM_MAX_RHS][MM_NVOI];

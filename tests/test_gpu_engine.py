@@ -9,9 +9,9 @@ import time
 import numpy as np
 import pytest
 
-from madmax_amd import (MM_ELL_DICT, MM_ELL_FULL, MM_PREC_FP64, MM_PREC_MIXED, MM_TANGENT_FD,
-                        MM_TANGENT_NONE, MM_TANGENT_UNIT, Engine, configs, fd_loads, rve_spec,
-                        unit_loads)
+from madmax_amd import (MM_ELL_DICT, MM_ELL_FULL, MM_GUESS_AFFINE, MM_GUESS_ZERO, MM_PREC_FP64,
+                        MM_PREC_MIXED, MM_TANGENT_FD, MM_TANGENT_NONE, MM_TANGENT_UNIT, Engine,
+                        configs, fd_loads, rve_spec, unit_loads)
 from madmax_amd import _lib
 from tests.helpers import normwise_rel, oracle_rve
 
@@ -237,3 +237,22 @@ def test_single_load_sequential_reuse(precision):
             print("load", i, r["status"], r["pcg_its"], r["stress"][0], out["stress_ave"], flush=True)
             assert r["status"] == 0
             assert normwise_rel(r["stress"][0], out["stress_ave"]) <= TOL
+
+
+@pytest.mark.parametrize("precision", [MM_PREC_FP64, MM_PREC_MIXED])
+def test_affine_initial_guess(precision):
+    """Starting the interior at the affine field reaches the same tangent (the
+    tolerance stays relative to the zero-start right-hand side) in fewer PCG
+    iterations than the reference's zero start."""
+    specs = [configs.c2_spec(2, i, n=14) for i in range(3)]
+    out = {}
+    for guess in (MM_GUESS_ZERO, MM_GUESS_AFFINE):
+        with Engine(nrhs=6, precision=precision, hbm_budget=GB, max_rves=8, initial_guess=guess,
+                    inner_rtol=1e-10 if precision == MM_PREC_FP64 else 1e-5) as eng:
+            out[guess] = _solve_batch(eng, specs)
+    for s, a, b in zip(specs, out[MM_GUESS_ZERO], out[MM_GUESS_AFFINE]):
+        c_ref = _oracle_tangent(s)
+        assert a["status"] == 0 and b["status"] == 0
+        assert normwise_rel(a["c_tangent"], c_ref) <= TOL
+        assert normwise_rel(b["c_tangent"], c_ref) <= TOL
+        assert b["pcg_its"] < a["pcg_its"]
