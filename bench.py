@@ -172,7 +172,7 @@ def main():
     ap.add_argument("--slab", type=int, default=0, help="nodes per ELL slab (64, 128, 256); 0 = autotune")
     ap.add_argument("--inner-rtol", type=float, default=0.0, help="FP32 PCG target per sweep; 0 = autotune")
     ap.add_argument("--rtol", type=float, default=1e-9, help="FP64 residual target (relative)")
-    ap.add_argument("--tune-batch", type=int, default=32)
+    ap.add_argument("--tune-batch", type=int, default=0, help="RVEs the autotuner solves (0 = the batch)")
     ap.add_argument("--spl", type=int, default=16, help="PCG steps per graph launch")
     ap.add_argument("--cpu-sample", type=int, default=2, help="RVE tangents timed on the CPU")
     ap.add_argument("--no-plain", action="store_true")
@@ -203,7 +203,7 @@ def main():
     if args.slab == 0 or args.inner_rtol == 0.0:
         from madmax_amd import autotune
         t = time.perf_counter()
-        tuned = autotune.tune(specs[: args.tune_batch], rtol=args.rtol, log=log, use_cache=False)
+        tuned = autotune.tune(specs[: args.tune_batch or B], rtol=args.rtol, log=log, use_cache=False)
         log("autotune (%.1f s): slab_nodes=%d inner_rtol=%g" % (time.perf_counter() - t,
                                                                 tuned["slab_nodes"], tuned["inner_rtol"]))
         if args.slab == 0:
