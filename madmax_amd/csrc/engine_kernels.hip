@@ -505,10 +505,16 @@ __device__ __forceinline__ T uniform_load(const T *p)
   return *(const __attribute__((address_space(4))) T *)p;
 }
 
+// Where a wavefront reads its dictionary blocks from.
+enum DictSrc : int {
+  DICT_SCALAR = 0,  // one entry for the whole wavefront: scalar unit
+  DICT_GLOBAL = 1,  // per lane, through L1
+  DICT_LDS = 2      // per lane, from the workgroup's LDS copy
+};
+
 // The 27-block product of an interior node from dictionary entry D, the
-// neighbours gathered in key order.  UNI: D is the same in every lane of the
-// wavefront and its values are read through the scalar unit.
-template <typename TS, int K, bool UNI>
+// neighbours gathered in key order.
+template <typename TS, int K, int SRC>
 __device__ __forceinline__ void dict_rows(const EngPtrs &P, const TS *D, int first_slab, int n, int nx, int nxy,
                                           const TS *__restrict__ z, TS (&w)[3][K], TS (&zc)[3][K])
 {
@@ -517,11 +523,17 @@ __device__ __forceinline__ void dict_rows(const EngPtrs &P, const TS *D, int fir
   for (int key = 0; key < 27; key++) {
     const int m = n + (key % 3 - 1) + ((key / 3) % 3 - 1) * nx + (key / 9 - 1) * nxy;
     TS v[3][4];  // [d'][d]
-    if (UNI) {
+    if (SRC == DICT_SCALAR) {
 #pragma unroll
       for (int dd = 0; dd < 3; dd++)
 #pragma unroll
         for (int d = 0; d < 3; d++) v[dd][d] = uniform_load(D + key * kDictBlk + dd * 4 + d);
+    } else if (SRC == DICT_LDS) {
+      const __attribute__((address_space(3))) TS *L = (const __attribute__((address_space(3))) TS *)D;
+#pragma unroll
+      for (int dd = 0; dd < 3; dd++)
+#pragma unroll
+        for (int d = 0; d < 4; d++) v[dd][d] = L[key * kDictBlk + dd * 4 + d];
     } else {
 #pragma unroll
       for (int dd = 0; dd < 3; dd++) load_vec<TS, 4>(D + key * kDictBlk + dd * 4, v[dd]);
@@ -543,18 +555,67 @@ __device__ __forceinline__ void dict_rows(const EngPtrs &P, const TS *D, int fir
   }
 }
 
+// The dictionary entries a workgroup's nodes use (at most `cap` distinct
+// patterns), copied to LDS once per workgroup so that wavefronts whose nodes
+// mix patterns read their blocks from LDS instead of through L1.
+template <typename TS>
+struct DictCache {
+  static constexpr int cap = (sizeof(TS) == 4) ? 16 : 8;
+  TS v[cap][kDictEntry];
+  int slot_of[kDictPat];  // LDS entry of a pattern, -1 if not cached
+  int pat_of[cap];
+  int count;
+};
+
+// Fills the cache; called by every thread of the workgroup.
+template <typename TS>
+__device__ void dict_cache_fill(const EngPtrs &P, const RveDev *R, int slot, int slab, DictCache<TS> &dc)
+{
+  const int tid = threadIdx.x;
+  for (int i = tid; i < kDictPat; i += blockDim.x) dc.slot_of[i] = 0;
+  __syncthreads();
+  const int n = P.slab_node0[slab] + tid;
+  const int nx = R->nx, nxy = R->nx * R->ny;
+  if (n < R->nrow / 3 && node_class(n % nx, (n / nx) % R->ny, n / nxy, nx, R->ny, R->nz) == 13)
+    dc.slot_of[P.npat[(long long)slab * P.SN + tid]] = 1;  // same value from every writer
+  __syncthreads();
+  if (tid < 64) {  // the first wavefront numbers the patterns in use
+    int base = 0;
+    for (int c = 0; c < kDictPat; c += 64) {
+      const bool used = dc.slot_of[c + tid] != 0;
+      const unsigned long long m = __ballot(used);
+      const int s = base + __popcll(m & ((1ull << tid) - 1ull));
+      int v = -1;
+      if (used && s < DictCache<TS>::cap) {
+        v = s;
+        dc.pat_of[s] = c + tid;
+      }
+      dc.slot_of[c + tid] = v;
+      base += __popcll(m);
+    }
+    if (tid == 0) dc.count = min(base, DictCache<TS>::cap);
+  }
+  __syncthreads();
+  const TS *D0 = (const TS *)P.dict + (size_t)slot * kDictPat * kDictEntry;
+  for (int e = tid; e < dc.count * kDictEntry; e += blockDim.x) {
+    const int s = e / kDictEntry, off = e - s * kDictEntry;
+    dc.v[s][off] = D0[(size_t)dc.pat_of[s] * kDictEntry + off];
+  }
+  __syncthreads();
+}
+
 // The same product from the dictionary: an interior node's 27 blocks come
 // from entry npat of its RVE at the structured neighbour offsets; a boundary
 // node's row is the identity, whose full-ELL product z + 0 is reproduced
 // exactly.  Same fma order as node_spmv (key order), so the same bits.  When every
-// interior lane of the wavefront has the same pattern (the common case) the
-// block values are read once per wavefront through the scalar unit instead of
-// once per lane through L1.  zc receives the node's own z rows (the centre
-// block of the gather).
+// interior lane of the wavefront has the same pattern the block values are
+// read once per wavefront through the scalar unit; otherwise from the
+// workgroup's LDS copy (dc), or through L1 if a pattern did not fit in it.
+// zc receives the node's own z rows (the centre block of the gather).
 template <typename TS, int K>
 __device__ __forceinline__ void node_spmv_dict(const EngPtrs &P, const RveDev *R, int slot, int slab, int tid,
-                                               int n, const TS *__restrict__ z, TS (&w)[3][K],
-                                               TS (&zc)[3][K])
+                                               int n, const TS *__restrict__ z, const DictCache<TS> &dc,
+                                               TS (&w)[3][K], TS (&zc)[3][K])
 {
   const int RS = 3 * P.SN;
   const int nx = R->nx, nxy = R->nx * R->ny;
@@ -578,18 +639,37 @@ __device__ __forceinline__ void node_spmv_dict(const EngPtrs &P, const RveDev *R
   const TS *D0 = (const TS *)P.dict + (size_t)slot * kDictPat * kDictEntry;
   const int pat = P.npat[(long long)slab * P.SN + tid];
   const int pat0 = __builtin_amdgcn_readfirstlane(pat);
+  const int cs = dc.slot_of[pat];
   if (__all(pat == pat0))
-    dict_rows<TS, K, true>(P, D0 + (size_t)pat0 * kDictEntry, R->first_slab, n, nx, nxy, z, w, zc);
+    dict_rows<TS, K, DICT_SCALAR>(P, D0 + (size_t)pat0 * kDictEntry, R->first_slab, n, nx, nxy, z, w, zc);
+  else if (__all(cs >= 0))
+    dict_rows<TS, K, DICT_LDS>(P, &dc.v[cs][0], R->first_slab, n, nx, nxy, z, w, zc);
   else
-    dict_rows<TS, K, false>(P, D0 + (size_t)pat * kDictEntry, R->first_slab, n, nx, nxy, z, w, zc);
+    dict_rows<TS, K, DICT_GLOBAL>(P, D0 + (size_t)pat * kDictEntry, R->first_slab, n, nx, nxy, z, w, zc);
 }
 
+// Either storage; dc is the workgroup's dictionary cache (dictionary storage
+// only, filled by dict_cache_fill).
 template <typename TS, int K, bool DICT>
 __device__ __forceinline__ void slab_node_spmv(const EngPtrs &P, const RveDev *R, int slot, int slab, int tid,
-                                               int n, TS (&w)[3][K], TS (&zc)[3][K])
+                                               int n, const DictCache<TS> *dc, TS (&w)[3][K], TS (&zc)[3][K])
 {
-  if (DICT) node_spmv_dict<TS, K>(P, R, slot, slab, tid, n, (const TS *)P.z, w, zc);
+  if (DICT) node_spmv_dict<TS, K>(P, R, slot, slab, tid, n, (const TS *)P.z, *dc, w, zc);
   else node_spmv<TS, K>(P, R->first_slab, slab, tid, n, (const TS *)P.z, w, zc);
+}
+
+// The workgroup's dictionary cache for storage DICT (nullptr for full
+// storage); every thread of the workgroup calls it.
+template <typename TS, bool DICT>
+__device__ __forceinline__ const DictCache<TS> *dict_cache(const EngPtrs &P, const RveDev *R, int slot, int slab)
+{
+  if constexpr (DICT) {
+    __shared__ DictCache<TS> dc;
+    dict_cache_fill<TS>(P, R, slot, slab, dc);
+    return &dc;
+  } else {
+    return nullptr;
+  }
 }
 
 // Dot products accumulate with explicit fma, so their rounding is the same in
@@ -762,9 +842,10 @@ __device__ __forceinline__ void inner_a(const EngPtrs &P, const RveDev *R, int s
   double v[K];
 #pragma unroll
   for (int k = 0; k < K; k++) v[k] = 0.0;
+  const DictCache<TS> *dc = dict_cache<TS, DICT>(P, R, slot, slab);
   if (n < R->nrow / 3) {
     TS w[3][K], zc[3][K];
-    slab_node_spmv<TS, K, DICT>(P, R, slot, slab, tid, n, w, zc);
+    slab_node_spmv<TS, K, DICT>(P, R, slot, slab, tid, n, dc, w, zc);
     TS *p = (TS *)P.p;
     TS *q = (TS *)P.q;
     const long long nb = sb + 3 * tid;
@@ -1122,9 +1203,10 @@ __global__ __launch_bounds__(256) void k_spmv_time(EngPtrs P)
   if (slot < 0) return;
   const RveDev *R = P.rves + slot;
   const int n = P.slab_node0[slab] + tid;
+  const DictCache<TS> *dc = dict_cache<TS, DICT>(P, R, slot, slab);
   if (n >= R->nrow / 3) return;
   TS w[3][K], zc[3][K];
-  slab_node_spmv<TS, K, DICT>(P, R, slot, slab, tid, n, w, zc);
+  slab_node_spmv<TS, K, DICT>(P, R, slot, slab, tid, n, dc, w, zc);
   scatter_node<TS, K>((TS *)P.q, node_vec(P, R->first_slab, n, K), 3 * P.SN, w);
 }
 
