@@ -505,6 +505,27 @@ __device__ __forceinline__ T uniform_load(const T *p)
   return *(const __attribute__((address_space(4))) T *)p;
 }
 
+// Four values of a 16-byte aligned LDS row in one read.
+typedef float lds_f4 __attribute__((ext_vector_type(4)));
+typedef double lds_d2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ void lds_row4(const float *p, float (&v)[4])
+{
+  const lds_f4 t = *(const __attribute__((address_space(3))) lds_f4 *)p;
+  v[0] = t.x;
+  v[1] = t.y;
+  v[2] = t.z;
+  v[3] = t.w;
+}
+__device__ __forceinline__ void lds_row4(const double *p, double (&v)[4])
+{
+  const __attribute__((address_space(3))) lds_d2 *q = (const __attribute__((address_space(3))) lds_d2 *)p;
+  const lds_d2 a = q[0], b = q[1];
+  v[0] = a.x;
+  v[1] = a.y;
+  v[2] = b.x;
+  v[3] = b.y;
+}
+
 // Where a wavefront reads its dictionary blocks from.
 enum DictSrc : int {
   DICT_SCALAR = 0,  // one entry for the whole wavefront: scalar unit
@@ -529,11 +550,8 @@ __device__ __forceinline__ void dict_rows(const EngPtrs &P, const TS *D, int fir
 #pragma unroll
         for (int d = 0; d < 3; d++) v[dd][d] = uniform_load(D + key * kDictBlk + dd * 4 + d);
     } else if (SRC == DICT_LDS) {
-      const __attribute__((address_space(3))) TS *L = (const __attribute__((address_space(3))) TS *)D;
 #pragma unroll
-      for (int dd = 0; dd < 3; dd++)
-#pragma unroll
-        for (int d = 0; d < 4; d++) v[dd][d] = L[key * kDictBlk + dd * 4 + d];
+      for (int dd = 0; dd < 3; dd++) lds_row4(D + key * kDictBlk + dd * 4, v[dd]);
     } else {
 #pragma unroll
       for (int dd = 0; dd < 3; dd++) load_vec<TS, 4>(D + key * kDictBlk + dd * 4, v[dd]);
@@ -559,9 +577,9 @@ __device__ __forceinline__ void dict_rows(const EngPtrs &P, const TS *D, int fir
 // patterns), copied to LDS once per workgroup so that wavefronts whose nodes
 // mix patterns read their blocks from LDS instead of through L1.
 template <typename TS>
-struct DictCache {
+struct alignas(16) DictCache {
   static constexpr int cap = (sizeof(TS) == 4) ? 16 : 8;
-  TS v[cap][kDictEntry];
+  alignas(16) TS v[cap][kDictEntry];
   int slot_of[kDictPat];  // LDS entry of a pattern, -1 if not cached
   int pat_of[cap];
   int count;
