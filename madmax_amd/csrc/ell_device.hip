@@ -30,8 +30,14 @@ struct IterState {
   double bnorm, rnorm, rz, alpha, beta, err;
   int its, done, max_its, pbuf;
   double rtol, min_err;
-  int cnt0, cnt1, pad0, pad1;
+  int pad0, pad1, pad2, pad3;
 };
+
+// Dot products: every workgroup of a big kernel writes its partial sums
+// (part[slab * 4 + i]); a one-workgroup kernel then sums them in slab order
+// and updates IterState, which the big kernels only read.  Kernel
+// boundaries order the two, so no workgroup fences for another.
+enum RedMode : int { RED_JAC = 0, RED_INIT = 1, RED_PQ = 2, RED_RR = 3 };
 
 __device__ inline double row_dot_exact(const int *__restrict__ cols, const double *__restrict__ vals,
                                        long long slab_base, int S, int nnz, int lane,
@@ -59,19 +65,59 @@ __global__ void k_spmv64_exact(const int *__restrict__ cols, const double *__res
   y[row] = row_dot_exact(cols, vals, base, S, nnz, lane, x);
 }
 
+// The one-workgroup reduction and the scalar steps of both solvers.
+__global__ __launch_bounds__(256) void k_reduce(const double *__restrict__ part, int nslab, int mode,
+                                                IterState *st)
+{
+  __shared__ double lds[48];
+  if (st->done) return;
+  if (mode == RED_JAC && st->its >= st->max_its) {  // loop condition of the reference
+    if (threadIdx.x == 0) st->done = 1;
+    return;
+  }
+  double s[3];
+  group_sum<3>(part, nslab, 4, 0, s, lds);
+  if (threadIdx.x != 0) return;
+  switch (mode) {
+    case RED_JAC:  // src/ell.c:160-180: err = ||A x - b||; stop, or update and count
+      st->err = sqrt(s[0]);
+      if (st->err < st->min_err) st->done = 1;
+      else st->its += 1;
+      break;
+    case RED_INIT:
+      st->bnorm = sqrt(s[0]);
+      st->rnorm = sqrt(s[1]);
+      st->rz = s[2];
+      st->beta = 0.0;
+      st->its = 0;
+      st->pbuf = 0;
+      st->done = (st->bnorm == 0.0 || st->rnorm <= st->rtol * st->bnorm) ? 1 : 0;
+      break;
+    case RED_PQ:
+      st->alpha = st->rz / s[0];
+      st->pbuf ^= 1;
+      break;
+    default:  // RED_RR
+      st->its += 1;
+      st->rnorm = sqrt(s[0]);
+      if (st->rnorm <= st->rtol * st->bnorm || st->its >= st->max_its) {
+        st->done = 1;
+      } else {
+        st->beta = s[1] / st->rz;
+        st->rz = s[1];
+      }
+  }
+}
+
 // ---- stationary Jacobi (src/ell.c:143-184) --------------------------------
-// k_jac_res: r = A x - b and ||r||^2; the last block decides convergence.
+// k_jac_res: r = A x - b and the partial ||r||^2 (part[.][0]).
 __global__ void k_jac_res(const int *__restrict__ cols, const double *__restrict__ vals, int nrow,
                           int nnz, int S, const double *__restrict__ x, const double *__restrict__ b,
-                          double *__restrict__ r, double *__restrict__ part, IterState *st)
+                          double *__restrict__ r, double *__restrict__ part, const IterState *st)
 {
 #pragma clang fp contract(off)
   __shared__ double lds[16];
-  if (st->done) return;
-  if (st->its >= st->max_its) {  // loop condition of the reference
-    if (blockIdx.x == 0 && threadIdx.x == 0) st->done = 1;
-    return;
-  }
+  if (st->done || st->its >= st->max_its) return;
   const int lane = threadIdx.x;
   const long long row = (long long)blockIdx.x * S + lane;
   double v[1] = {0.0};
@@ -83,13 +129,10 @@ __global__ void k_jac_res(const int *__restrict__ cols, const double *__restrict
     v[0] = ri * ri;
   }
   block_sum<1>(v, lds);
-  if (threadIdx.x == 0) part[blockIdx.x] = v[0];
-  if (arrive_last(&st->cnt0, gridDim.x) && threadIdx.x == 0) {
-    double s = 0.0;
-    for (int i = 0; i < (int)gridDim.x; i++) s += load_coherent(&part[i]);
-    st->err = sqrt(s);
-    if (st->err < st->min_err) st->done = 1;
-    else st->its += 1;
+  if (threadIdx.x == 0) {
+    part[blockIdx.x * 4] = v[0];
+    part[blockIdx.x * 4 + 1] = 0.0;
+    part[blockIdx.x * 4 + 2] = 0.0;
   }
 }
 
@@ -106,12 +149,12 @@ __global__ void k_jac_upd(int nrow, int S, const double *__restrict__ dinv,
 }
 
 // ---- Jacobi-preconditioned CG in FP64 ---------------------------------------
-// init: r = b - A x, z = dinv r, ||b||, ||r||, r.z
+// init: r = b - A x, z = dinv r; partials ||b||^2, ||r||^2, r.z
 __global__ void k_pcg_init(const int *__restrict__ cols, const double *__restrict__ vals, int nrow,
                            int nnz, int S, const double *__restrict__ x, const double *__restrict__ b,
                            const double *__restrict__ dinv, double *__restrict__ r,
                            double *__restrict__ z, double *__restrict__ p0, double *__restrict__ p1,
-                           double *__restrict__ part, IterState *st)
+                           double *__restrict__ part)
 {
   __shared__ double lds[48];
   const int lane = threadIdx.x;
@@ -134,26 +177,14 @@ __global__ void k_pcg_init(const int *__restrict__ cols, const double *__restric
   block_sum<3>(v, lds);
   if (threadIdx.x == 0)
     for (int i = 0; i < 3; i++) part[blockIdx.x * 4 + i] = v[i];
-  if (arrive_last(&st->cnt0, gridDim.x) && threadIdx.x == 0) {
-    double s[3] = {0.0, 0.0, 0.0};
-    for (int k = 0; k < (int)gridDim.x; k++)
-      for (int i = 0; i < 3; i++) s[i] += load_coherent(&part[k * 4 + i]);
-    st->bnorm = sqrt(s[0]);
-    st->rnorm = sqrt(s[1]);
-    st->rz = s[2];
-    st->beta = 0.0;
-    st->its = 0;
-    st->pbuf = 0;
-    st->done = (st->bnorm == 0.0 || st->rnorm <= st->rtol * st->bnorm) ? 1 : 0;
-  }
 }
 
 // q = A p with p = z + beta p_old formed on the fly (both for the own rows,
-// which are stored, and for every gathered column), then alpha = r.z / p.q
+// which are stored, and for every gathered column); partial p.q (part[.][0])
 __global__ void k_pcg_spmv(const int *__restrict__ cols, const double *__restrict__ vals, int nrow,
                            int nnz, int S, const double *__restrict__ z, double *__restrict__ pA,
                            double *__restrict__ pB, double *__restrict__ q,
-                           double *__restrict__ part, IterState *st)
+                           double *__restrict__ part, const IterState *st)
 {
   __shared__ double lds[16];
   if (st->done) return;
@@ -177,21 +208,19 @@ __global__ void k_pcg_spmv(const int *__restrict__ cols, const double *__restric
     v[0] = pi * acc;
   }
   block_sum<1>(v, lds);
-  if (threadIdx.x == 0) part[blockIdx.x * 4] = v[0];
-  if (arrive_last(&st->cnt0, gridDim.x) && threadIdx.x == 0) {
-    double s = 0.0;
-    for (int k = 0; k < (int)gridDim.x; k++) s += load_coherent(&part[k * 4]);
-    st->alpha = st->rz / s;
-    st->pbuf ^= 1;
+  if (threadIdx.x == 0) {
+    part[blockIdx.x * 4] = v[0];
+    part[blockIdx.x * 4 + 1] = 0.0;
+    part[blockIdx.x * 4 + 2] = 0.0;
   }
 }
 
-// x += alpha p, r -= alpha q, z = dinv r, then r.r and r.z
+// x += alpha p, r -= alpha q, z = dinv r; partials r.r, r.z (part[.][0..1])
 __global__ void k_pcg_update(int nrow, int S, const double *__restrict__ dinv,
                              const double *__restrict__ pA, const double *__restrict__ pB,
                              const double *__restrict__ q, double *__restrict__ x,
                              double *__restrict__ r, double *__restrict__ z,
-                             double *__restrict__ part, IterState *st)
+                             double *__restrict__ part, const IterState *st)
 {
   __shared__ double lds[32];
   if (st->done) return;
@@ -211,23 +240,9 @@ __global__ void k_pcg_update(int nrow, int S, const double *__restrict__ dinv,
   }
   block_sum<2>(v, lds);
   if (threadIdx.x == 0) {
-    part[blockIdx.x * 4 + 1] = v[0];
-    part[blockIdx.x * 4 + 2] = v[1];
-  }
-  if (arrive_last(&st->cnt1, gridDim.x) && threadIdx.x == 0) {
-    double s[2] = {0.0, 0.0};
-    for (int k = 0; k < (int)gridDim.x; k++) {
-      s[0] += load_coherent(&part[k * 4 + 1]);
-      s[1] += load_coherent(&part[k * 4 + 2]);
-    }
-    st->its += 1;
-    st->rnorm = sqrt(s[0]);
-    if (st->rnorm <= st->rtol * st->bnorm || st->its >= st->max_its) {
-      st->done = 1;
-    } else {
-      st->beta = s[1] / st->rz;
-      st->rz = s[1];
-    }
+    part[blockIdx.x * 4] = v[0];
+    part[blockIdx.x * 4 + 1] = v[1];
+    part[blockIdx.x * 4 + 2] = 0.0;
   }
 }
 
@@ -405,6 +420,7 @@ extern "C" int ell_device_solve_jacobi(ell_device *d, ell_solver *solver, const 
     for (int k = 0; k < 64; k++) {
       k_jac_res<<<d->nslab, d->S, 0, d->stream>>>(d->cols, d->vals, d->nrow, d->nnz, d->S, d->x, d->b,
                                                   d->r, d->part, d->st);
+      k_reduce<<<1, 256, 0, d->stream>>>(d->part, d->nslab, RED_JAC, d->st);
       k_jac_upd<<<d->nslab, d->S, 0, d->stream>>>(d->nrow, d->S, d->dinv, d->r, d->x, d->st);
     }
     MM_HIP(hipGetLastError());
@@ -433,7 +449,8 @@ extern "C" int ell_device_solve_pcg(ell_device *d, ell_solver *solver, const dou
   MM_HIP(hipMemcpyAsync(d->x, x, d->nrow * sizeof(double), hipMemcpyHostToDevice, d->stream));
   MM_HIP(hipMemcpyAsync(d->b, b, d->nrow * sizeof(double), hipMemcpyHostToDevice, d->stream));
   k_pcg_init<<<d->nslab, d->S, 0, d->stream>>>(d->cols, d->vals, d->nrow, d->nnz, d->S, d->x, d->b,
-                                               d->dinv, d->r, d->z, d->pA, d->pB, d->part, d->st);
+                                               d->dinv, d->r, d->z, d->pA, d->pB, d->part);
+  k_reduce<<<1, 256, 0, d->stream>>>(d->part, d->nslab, RED_INIT, d->st);
   MM_HIP(hipGetLastError());
   IterState h;
   rc = get_state(d, &h);
@@ -445,8 +462,10 @@ extern "C" int ell_device_solve_pcg(ell_device *d, ell_solver *solver, const dou
     for (int k = 0; k < 32; k++) {
       k_pcg_spmv<<<d->nslab, d->S, 0, d->stream>>>(d->cols, d->vals, d->nrow, d->nnz, d->S, d->z,
                                                    d->pA, d->pB, d->q, d->part, d->st);
+      k_reduce<<<1, 256, 0, d->stream>>>(d->part, d->nslab, RED_PQ, d->st);
       k_pcg_update<<<d->nslab, d->S, 0, d->stream>>>(d->nrow, d->S, d->dinv, d->pA, d->pB, d->q, d->x,
                                                      d->r, d->z, d->part, d->st);
+      k_reduce<<<1, 256, 0, d->stream>>>(d->part, d->nslab, RED_RR, d->st);
     }
     MM_HIP(hipGetLastError());
     rc = get_state(d, &h);
