@@ -256,3 +256,45 @@ def test_affine_initial_guess(precision):
         assert normwise_rel(a["c_tangent"], c_ref) <= TOL
         assert normwise_rel(b["c_tangent"], c_ref) <= TOL
         assert b["pcg_its"] < a["pcg_its"]
+
+
+def test_large_rve_mixed_vs_fp64():
+    """One 56^3 RVE (0.55 M dofs, 4 fibers): the mixed-precision stress equals
+    the FP64 solve within the bar, with either matrix storage (bit-identical
+    to each other)."""
+    spec = configs.c3_spec(n=56)
+    eps = configs.c3_strain()
+    out = {}
+    for name, prec, storage in (("mixed_dict", MM_PREC_MIXED, MM_ELL_DICT),
+                                ("mixed_full", MM_PREC_MIXED, MM_ELL_FULL),
+                                ("fp64", MM_PREC_FP64, MM_ELL_DICT)):
+        with Engine(nrhs=1, precision=prec, hbm_budget=2 * GB, max_rves=2, slab_nodes=256, rtol=1e-11,
+                    inner_rtol=1e-5 if prec == MM_PREC_MIXED else 1e-11, max_inner_its=100000,
+                    max_sweeps=12 if prec == MM_PREC_MIXED else 2, ell_storage=storage) as eng:
+            assert eng.submit(1, spec, [eps], MM_TANGENT_NONE) == 0
+            eng.drain()
+            out[name] = eng.poll()[0]
+        assert out[name]["status"] == 0
+    assert np.array_equal(out["mixed_dict"]["stress"][0], out["mixed_full"]["stress"][0])
+    assert normwise_rel(out["mixed_dict"]["stress"][0], out["fp64"]["stress"][0]) <= TOL
+
+
+def test_lru_eviction_keeps_results_exact():
+    """More warm-keyed RVEs than fit: the least recently used resident RVEs
+    are evicted to admit new ones, and a key that comes back after eviction
+    is assembled again and gives the same tangent."""
+    specs = [rve_spec(10, [(0.5, 0.5, 0.2 + 0.01 * i)]) for i in range(6)]
+    with Engine(nrhs=6, hbm_budget=48 << 20, max_rves=3) as eng:
+        first = {}
+        for i, s in enumerate(specs):
+            assert eng.submit(i + 1, s, unit_loads(), MM_TANGENT_UNIT, warm_key=100 + i) == 0
+            eng.drain()
+            first[i] = eng.poll()[0]
+        info = eng.info()
+        assert info["resident"] <= 3
+        # key 100 was evicted; resubmitting it assembles it again
+        assert eng.submit(50, specs[0], unit_loads(), MM_TANGENT_UNIT, warm_key=100) == 0
+        eng.drain()
+        again = eng.poll()[0]
+    assert all(r["status"] == 0 for r in first.values())
+    assert np.array_equal(again["c_tangent"], first[0]["c_tangent"])
