@@ -160,7 +160,7 @@ struct mm_engine {
   int *bcols = nullptr;
   void *vals = nullptr, *dinv = nullptr;
   double *x64 = nullptr;
-  void *d = nullptr, *r = nullptr, *z = nullptr, *p = nullptr, *q = nullptr;
+  void *d = nullptr, *z = nullptr, *p = nullptr, *q = nullptr;
   double *part = nullptr;
   int *slab_rve = nullptr, *slab_node0 = nullptr;
   RveDev *rves = nullptr;
@@ -218,7 +218,6 @@ struct mm_engine {
     P.dinv = dinv;
     P.x64 = x64;
     P.d = d;
-    P.r = r;
     P.z = z;
     P.p = p;
     P.q = q;
@@ -260,13 +259,13 @@ struct mm_engine {
   {
     const size_t RS = (size_t)rows_per_slab();
     const size_t ell = dict_mode ? 0 : RS * kW * tsz + (size_t)SN * 27 * 4;
-    return ell + RS * (K * (8 + 5 * tsz) + tsz) + kPartDoubles * 8 + 8 + (size_t)SN * 2 + 1;
+    return ell + RS * (K * (8 + 4 * tsz) + tsz) + kPartDoubles * 8 + 8 + (size_t)SN * 2 + 1;
   }
 };
 
 static int engine_free_arena(mm_engine *e)
 {
-  void *ptrs[] = {e->bcols, e->vals, e->dinv, e->x64, e->d, e->r, e->z, e->p, e->q, e->part,
+  void *ptrs[] = {e->bcols, e->vals, e->dinv, e->x64, e->d, e->z, e->p, e->q, e->part,
                   e->slab_rve, e->slab_node0, e->rves, e->elem_mat, e->npat,
                   (e->dict != (void *)e->dict64) ? e->dict : nullptr, e->dict64, e->ke_tab, e->c_tab,
                   e->bw_tab,
@@ -392,7 +391,6 @@ extern "C" int mm_engine_create(const mm_engine_config *cfg, mm_engine **out)
   EALLOC(e->dinv, nrows * e->tsz);
   EALLOC(e->x64, nrows * e->K * 8);
   EALLOC(e->d, nrows * e->K * e->tsz);
-  EALLOC(e->r, nrows * e->K * e->tsz);
   EALLOC(e->z, nrows * e->K * e->tsz);
   EALLOC(e->p, nrows * e->K * e->tsz);
   EALLOC(e->q, nrows * e->K * e->tsz);

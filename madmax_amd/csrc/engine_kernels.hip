@@ -960,8 +960,9 @@ __global__ __launch_bounds__(256) void k_step_a(EngPtrs P, SolveParams sp)
 // ---------------------------------------------------------------------------
 // Kernel B (reads st[0], writes st[1]).  PH_INNER: alpha = r.z / p.q from
 // the last A's partials, then over the slab's values d += alpha p,
-// r -= alpha q, z = dinv r and the partial r.r, r.z, for the load cases that
-// are still iterating.
+// z -= alpha dinv q and the partial r.r, r.z, for the load cases that are
+// still iterating.  The residual itself is not stored: z = dinv r carries it
+// and r = z / dinv is formed in registers for the dot products.
 template <typename TS, int K>
 __global__ __launch_bounds__(256) void k_step_b(EngPtrs P, SolveParams sp)
 {
@@ -990,7 +991,7 @@ __global__ __launch_bounds__(256) void k_step_b(EngPtrs P, SolveParams sp)
   const int npair = rows >> 1;
   const long long sb = (long long)slab * K * RS;
   const TS *dinv = (const TS *)P.dinv + (long long)slab * RS;
-  TS *dv = (TS *)P.d, *rv = (TS *)P.r, *zv = (TS *)P.z;
+  TS *dv = (TS *)P.d, *zv = (TS *)P.z;
   const TS *p = (const TS *)P.p, *q = (const TS *)P.q;
   double vu[2 * K];
 #pragma unroll
@@ -1004,32 +1005,29 @@ __global__ __launch_bounds__(256) void k_step_b(EngPtrs P, SolveParams sp)
       const long long i = kb + 2 * g;
       const T2 dg = *reinterpret_cast<const T2 *>(dinv + 2 * g);
       T2 di = *reinterpret_cast<const T2 *>(dv + i);
-      T2 ri = *reinterpret_cast<const T2 *>(rv + i);
+      T2 zi = *reinterpret_cast<const T2 *>(zv + i);
       const T2 pi = *reinterpret_cast<const T2 *>(p + i);
       const T2 qi = *reinterpret_cast<const T2 *>(q + i);
       di.x = fma(a, pi.x, di.x);
       di.y = fma(a, pi.y, di.y);
-      ri.x = fma(-a, qi.x, ri.x);
-      ri.y = fma(-a, qi.y, ri.y);
-      T2 zi;
-      zi.x = dg.x * ri.x;
-      zi.y = dg.y * ri.y;
+      zi.x = fma(-a, dg.x * qi.x, zi.x);
+      zi.y = fma(-a, dg.y * qi.y, zi.y);
       *reinterpret_cast<T2 *>(dv + i) = di;
-      *reinterpret_cast<T2 *>(rv + i) = ri;
       *reinterpret_cast<T2 *>(zv + i) = zi;
-      vu[k] = fma((double)ri.x, (double)ri.x, vu[k]);
-      vu[k] = fma((double)ri.y, (double)ri.y, vu[k]);
-      vu[K + k] = fma((double)ri.x, (double)zi.x, vu[K + k]);
-      vu[K + k] = fma((double)ri.y, (double)zi.y, vu[K + k]);
+      const TS rx = zi.x / dg.x, ry = zi.y / dg.y;  // r = z / dinv
+      vu[k] = fma((double)rx, (double)rx, vu[k]);
+      vu[k] = fma((double)ry, (double)ry, vu[k]);
+      vu[K + k] = fma((double)rx, (double)zi.x, vu[K + k]);
+      vu[K + k] = fma((double)ry, (double)zi.y, vu[K + k]);
     }
     if ((rows & 1) && tid == 0) {
       const long long i = kb + rows - 1;
+      const TS dg = dinv[rows - 1];
       const TS di = fma(a, p[i], dv[i]);
-      const TS ri = fma(-a, q[i], rv[i]);
-      const TS zi = dinv[rows - 1] * ri;
+      const TS zi = fma(-a, dg * q[i], zv[i]);
       dv[i] = di;
-      rv[i] = ri;
       zv[i] = zi;
+      const TS ri = zi / dg;
       vu[k] = fma((double)ri, (double)ri, vu[k]);
       vu[K + k] = fma((double)ri, (double)zi, vu[K + k]);
     }
@@ -1124,7 +1122,7 @@ __device__ void res_slab(const EngPtrs &P, int slab, int slot, const RveDev *R, 
         v[3 * K + k] = fma(rc64[d][k], rc64[d][k], v[3 * K + k]);
       }
     }
-    scatter_node<TS, K>((TS *)P.r, nb, RS, ri);
+
     scatter_node<TS, K>((TS *)P.z, nb, RS, zi);
     scatter_node<TS, K>((TS *)P.d, nb, RS, zero);
     scatter_node<TS, K>((TS *)P.p, nb, RS, zero);

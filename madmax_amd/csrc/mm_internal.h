@@ -71,7 +71,8 @@ __host__ __device__ inline void local_coords(int a, int *lx, int *ly, int *lz)
 //      PH_INNER:  reduce r.r and r.z of the last B; converged -> x += d and
 //                 PH_RES; else SpMV, p, q and partial p.q
 //      PH_AVGRED: reduce the averaging partials, write the result, PH_DONE
-//   B  PH_INNER:  reduce p.q, alpha; d, r, z and partial r.r, r.z
+//   B  PH_INNER:  reduce p.q, alpha; d, z (z = dinv r; r is not stored) and
+//                 partial r.r, r.z
 //   R  PH_RES:    FP64 residual, partial norms -> PH_RESRED
 //      PH_AVG:    partial volume averages -> PH_AVGRED
 // Every reduction is done by each workgroup of the consuming launch over the
@@ -139,7 +140,7 @@ struct EngPtrs {
   const void *vals;  // [slab][81][3*SN]
   const void *dinv;
   double *x64;
-  void *d, *r, *z, *p, *q;
+  void *d, *z, *p, *q;  // residual kept only as z = dinv r
   double *part;
   const int *slab_rve, *slab_node0;
   RveDev *rves;
