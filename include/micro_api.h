@@ -171,6 +171,8 @@ typedef struct {
   int sync_every_step;   /* plain path: host reads the state after every step */
   int ell_storage;       /* MM_ELL_FULL or MM_ELL_DICT (default DICT) */
   int initial_guess;     /* MM_GUESS_ZERO or MM_GUESS_AFFINE (default AFFINE) */
+  int solver_streams;    /* 1, or 2 (default): resident RVEs stepped in two
+                            groups on two HIP streams */
 } mm_engine_config;
 
 typedef struct {

@@ -77,7 +77,7 @@ class EngineConfig(C.Structure):
                 ("max_inner_its", C.c_int), ("max_sweeps", C.c_int),
                 ("steps_per_launch", C.c_int), ("use_graph", C.c_int),
                 ("sync_every_step", C.c_int), ("ell_storage", C.c_int),
-                ("initial_guess", C.c_int)]
+                ("initial_guess", C.c_int), ("solver_streams", C.c_int)]
 
 
 class Request(C.Structure):

@@ -154,6 +154,13 @@ struct mm_engine {
   bool dict_mode = false;  // MM_ELL_DICT: no per-row ELL arrays, dictionary + pattern bytes
   size_t tsz = 4;
   hipStream_t stream = nullptr;
+  // solver_streams == 2: the resident RVEs are split into two groups of
+  // contiguous slabs, each stepped on its own stream (stream, stream2), so the
+  // gather-bound kernel A of one group runs beside the streaming kernel B of
+  // the other.  ev_fork / ev_join order stream2 after the admission work and
+  // the engine stream after stream2's steps.
+  hipStream_t stream2 = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   // arena
   long long nslab_cap = 0, elem_cap = 0;
   int rve_cap = 0;
@@ -192,15 +199,17 @@ struct mm_engine {
   std::condition_variable qcv;
   std::deque<Queued> queue;
   std::deque<mm_result> results;
-  // solver launches run over the slabs of the RVEs being solved: act_list[0]
-  // is their count, act_list[1..] the slabs in ascending order (device), staged
-  // in st_act (pinned); rebuilt when an RVE is admitted or retired
+  // solver launches run over the slabs of the RVEs being solved, one list
+  // per stream group g at act_list + g * (nslab_cap + 1): [0] is the count,
+  // [1..] the slabs in ascending order (device), staged in st_act (pinned);
+  // rebuilt when an RVE is admitted or retired
   int *act_list = nullptr, *st_act = nullptr;
   bool act_dirty = true;
-  long long act_count = 0;
-  // one captured graph per launch grid (a power of two >= 64 covering the
-  // active slabs); the kernels read the active count from act_list[0]
-  std::map<int, std::pair<hipGraph_t, hipGraphExec_t>> graphs;
+  long long act_n[2] = {0, 0};
+  // one captured graph per pair of launch grids (powers of two >= 64 covering
+  // each group's slabs, 0 for an empty group); the kernels read the active
+  // count from their list
+  std::map<std::pair<int, int>, std::pair<hipGraph_t, hipGraphExec_t>> graphs;
   // stats / threading
   std::mutex step_mu;
   long long completed = 0, nsteps = 0;
@@ -231,7 +240,7 @@ struct mm_engine {
     P.bw_tab = bw_tab;
     P.elem_mat = elem_mat;
     P.npat = npat;
-    P.act_list = act_list;
+    P.act_list = act_list;  // group 0; run_steps points group 1 at its list
     P.dict = dict;
     P.dict64 = dict64;
     P.SN = SN;
@@ -262,6 +271,16 @@ struct mm_engine {
     return ell + RS * (K * (8 + 4 * tsz) + tsz) + kPartDoubles * 8 + 8 + (size_t)SN * 2 + 1;
   }
 };
+
+static void destroy_streams(mm_engine *e)
+{
+  if (e->ev_fork) (void)hipEventDestroy(e->ev_fork);
+  if (e->ev_join) (void)hipEventDestroy(e->ev_join);
+  if (e->stream2) (void)hipStreamDestroy(e->stream2);
+  if (e->stream) (void)hipStreamDestroy(e->stream);
+  e->ev_fork = e->ev_join = nullptr;
+  e->stream2 = e->stream = nullptr;
+}
 
 static int engine_free_arena(mm_engine *e)
 {
@@ -299,6 +318,7 @@ extern "C" int mm_engine_config_default(mm_engine_config *c)
   c->sync_every_step = 0;
   c->ell_storage = MM_ELL_DICT;
   c->initial_guess = MM_GUESS_AFFINE;
+  c->solver_streams = 2;
   return 0;
 }
 
@@ -314,6 +334,7 @@ extern "C" int mm_engine_create(const mm_engine_config *cfg, mm_engine **out)
   if (cfg->precision != MM_PREC_FP64 && cfg->precision != MM_PREC_MIXED) return MM_ERR_ARG;
   if (cfg->ell_storage != MM_ELL_FULL && cfg->ell_storage != MM_ELL_DICT) return MM_ERR_ARG;
   if (cfg->initial_guess != MM_GUESS_ZERO && cfg->initial_guess != MM_GUESS_AFFINE) return MM_ERR_ARG;
+  if (cfg->solver_streams != 1 && cfg->solver_streams != 2) return MM_ERR_ARG;
   if (!(cfg->rtol >= 0) || !(cfg->inner_rtol > 0) || cfg->max_inner_its < 1 || cfg->max_sweeps < 1)
     return MM_ERR_ARG;
   int ndev = 0;
@@ -367,6 +388,14 @@ extern "C" int mm_engine_create(const mm_engine_config *cfg, mm_engine **out)
     delete e;
     return MM_ERR_HIP;
   }
+  if (e->cfg.solver_streams == 2 &&
+      (hipStreamCreateWithFlags(&e->stream2, hipStreamNonBlocking) != hipSuccess ||
+       hipEventCreateWithFlags(&e->ev_fork, hipEventDisableTiming) != hipSuccess ||
+       hipEventCreateWithFlags(&e->ev_join, hipEventDisableTiming) != hipSuccess)) {
+    destroy_streams(e);
+    delete e;
+    return MM_ERR_HIP;
+  }
 #define EALLOC(ptr, bytes)                                                  \
   do {                                                                      \
     if (hipMalloc((void **)&(ptr), (bytes)) != hipSuccess) {                \
@@ -387,7 +416,7 @@ extern "C" int mm_engine_create(const mm_engine_config *cfg, mm_engine **out)
     else EALLOC(e->dict, (size_t)e->rve_cap * kDictPat * kDictEntry * 4);
   }
   EALLOC(e->npat, (size_t)e->nslab_cap * e->SN);
-  EALLOC(e->act_list, ((size_t)e->nslab_cap + 1) * 4);
+  EALLOC(e->act_list, 2 * ((size_t)e->nslab_cap + 1) * 4);
   EALLOC(e->dinv, nrows * e->tsz);
   EALLOC(e->x64, nrows * e->K * 8);
   EALLOC(e->d, nrows * e->K * e->tsz);
@@ -409,7 +438,8 @@ extern "C" int mm_engine_create(const mm_engine_config *cfg, mm_engine **out)
     goto fail;
   }
   memset(e->out_host, 0, (size_t)e->rve_cap * sizeof(RveOut));
-  if (hipHostMalloc((void **)&e->st_act, ((size_t)e->nslab_cap + 1) * 4, hipHostMallocDefault) != hipSuccess) {
+  if (hipHostMalloc((void **)&e->st_act, 2 * ((size_t)e->nslab_cap + 1) * 4, hipHostMallocDefault) !=
+      hipSuccess) {
     rc = MM_ERR_HIP;
     goto fail;
   }
@@ -432,7 +462,7 @@ extern "C" int mm_engine_create(const mm_engine_config *cfg, mm_engine **out)
   return 0;
 fail:
   engine_free_arena(e);
-  if (e->stream) (void)hipStreamDestroy(e->stream);
+  destroy_streams(e);
   delete e;
   return rc;
 }
@@ -449,7 +479,7 @@ extern "C" int mm_engine_destroy(mm_engine *e)
     (void)hipGraphDestroy(g.second.first);
   }
   engine_free_arena(e);
-  if (e->stream) (void)hipStreamDestroy(e->stream);
+  destroy_streams(e);
   delete e;
   return 0;
 }
@@ -698,23 +728,74 @@ static int solver_grid(const mm_engine *e, long long hw)
   return (int)g;
 }
 
-// Rebuilds the list of slabs being solved (ascending, so each XCD walks
-// contiguous slabs) and queues its upload on the engine stream.  The pinned
-// staging buffer is free: the previous step ended with a stream sync.
+// Rebuilds the lists of slabs being solved (ascending, so each XCD walks
+// contiguous slabs) and queues their upload on the engine stream.  With two
+// solver streams the RVEs, in slab order, are cut where group 0 first holds
+// at least half of the slabs.  The pinned staging buffer is free: the
+// previous step ended with a stream sync.
 static int update_active(mm_engine *e)
 {
   if (!e->act_dirty) return 0;
   std::vector<std::pair<long long, long long>> runs;
+  long long total = 0;
   for (int s = 0; s < e->rve_cap; s++)
-    if (e->slots[s].state == SL_ACTIVE) runs.push_back({e->slots[s].first_slab, e->slots[s].nslab});
+    if (e->slots[s].state == SL_ACTIVE) {
+      runs.push_back({e->slots[s].first_slab, e->slots[s].nslab});
+      total += e->slots[s].nslab;
+    }
   std::sort(runs.begin(), runs.end());
-  long long n = 0;
-  for (auto &r : runs)
-    for (long long b = 0; b < r.second; b++) e->st_act[1 + n++] = (int)(r.first + b);
-  e->st_act[0] = (int)n;
-  MM_HIP(hipMemcpyAsync(e->act_list, e->st_act, (size_t)(n + 1) * 4, hipMemcpyHostToDevice, e->stream));
-  e->act_count = n;
+  const size_t stride = (size_t)e->nslab_cap + 1;
+  const int ngroup = e->stream2 ? 2 : 1;
+  long long n[2] = {0, 0};
+  int g = 0;
+  for (auto &r : runs) {
+    if (g == 0 && ngroup == 2 && n[0] > 0 && 2 * n[0] >= total) g = 1;
+    for (long long b = 0; b < r.second; b++) e->st_act[g * stride + 1 + n[g]++] = (int)(r.first + b);
+  }
+  for (int q = 0; q < 2; q++) {
+    e->st_act[q * stride] = (int)n[q];
+    if (q < ngroup)
+      MM_HIP(hipMemcpyAsync(e->act_list + q * stride, e->st_act + q * stride, (size_t)(n[q] + 1) * 4,
+                            hipMemcpyHostToDevice, e->stream));
+    e->act_n[q] = n[q];
+  }
   e->act_dirty = false;
+  return 0;
+}
+
+static int launch_grid(long long n)
+{
+  if (n <= 0) return 0;
+  int grid = 64;
+  while (grid < n) grid *= 2;
+  return grid;
+}
+
+// Queues steps_per_launch solver steps (A, B, R each) for every RVE being
+// solved: group 0 on the engine stream; group 1, if any, on stream2, forked
+// after group 0's first kernel A so that the two chains start one kernel
+// apart (A of one group tends to run beside B of the other), and joined
+// before the engine stream's other work.
+static int queue_steps(mm_engine *e, int grid0, int grid1)
+{
+  EngPtrs P0 = e->ptrs(), P1 = P0;
+  P1.act_list = e->act_list + ((size_t)e->nslab_cap + 1);
+  const SolveParams sp = e->params();
+  const int nk = 3 * e->cfg.steps_per_launch;
+  const bool f64 = e->fp64, dm = e->dict_mode;
+  int rc = 0;
+  if (grid1 == 0) return eng_launch_steps(f64, e->K, dm, P0, sp, grid0, 0, nk, e->stream);
+  int first = 0;
+  if (grid0 > 0) {
+    if ((rc = eng_launch_steps(f64, e->K, dm, P0, sp, grid0, 0, 1, e->stream))) return rc;
+    first = 1;
+  }
+  MM_HIP(hipEventRecord(e->ev_fork, e->stream));
+  MM_HIP(hipStreamWaitEvent(e->stream2, e->ev_fork, 0));
+  if (grid0 > 0 && (rc = eng_launch_steps(f64, e->K, dm, P0, sp, grid0, first, nk - first, e->stream))) return rc;
+  if ((rc = eng_launch_steps(f64, e->K, dm, P1, sp, grid1, 0, nk, e->stream2))) return rc;
+  MM_HIP(hipEventRecord(e->ev_join, e->stream2));
+  MM_HIP(hipStreamWaitEvent(e->stream, e->ev_join, 0));
   return 0;
 }
 
@@ -722,33 +803,29 @@ static int run_steps(mm_engine *e)
 {
   int rc = update_active(e);
   if (rc) return rc;
-  if (e->act_count <= 0) return 0;
-  int grid = 64;
-  while (grid < e->act_count) grid *= 2;
-  EngPtrs P = e->ptrs();
-  SolveParams sp = e->params();
-  const int T = e->cfg.steps_per_launch;
-  if (e->cfg.use_graph) {
-    auto it = e->graphs.find(grid);
-    if (it == e->graphs.end()) {
-      MM_HIP(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
-      rc = eng_launch_steps(e->fp64, e->K, e->dict_mode, P, sp, grid, T, e->stream);
-      hipGraph_t g = nullptr;
-      hipError_t ce = hipStreamEndCapture(e->stream, &g);
-      if (rc) return rc;
-      MM_HIP(ce);
-      hipGraphExec_t x = nullptr;
-      if (hipGraphInstantiate(&x, g, nullptr, nullptr, 0) != hipSuccess) {
-        (void)hipGraphDestroy(g);
-        return MM_ERR_HIP;
-      }
-      it = e->graphs.emplace(grid, std::make_pair(g, x)).first;
+  const int grid0 = launch_grid(e->act_n[0]), grid1 = launch_grid(e->act_n[1]);
+  if (grid0 == 0 && grid1 == 0) return 0;
+  if (!e->cfg.use_graph) return queue_steps(e, grid0, grid1);
+  const std::pair<int, int> key(grid0, grid1);
+  auto it = e->graphs.find(key);
+  if (it == e->graphs.end()) {
+    MM_HIP(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
+    rc = queue_steps(e, grid0, grid1);
+    hipGraph_t g = nullptr;
+    hipError_t ce = hipStreamEndCapture(e->stream, &g);
+    if (rc) {
+      if (g) (void)hipGraphDestroy(g);
+      return rc;
     }
-    MM_HIP(hipGraphLaunch(it->second.second, e->stream));
-  } else {
-    rc = eng_launch_steps(e->fp64, e->K, e->dict_mode, P, sp, grid, T, e->stream);
-    if (rc) return rc;
+    MM_HIP(ce);
+    hipGraphExec_t x = nullptr;
+    if (hipGraphInstantiate(&x, g, nullptr, nullptr, 0) != hipSuccess) {
+      (void)hipGraphDestroy(g);
+      return MM_ERR_HIP;
+    }
+    it = e->graphs.emplace(key, std::make_pair(g, x)).first;
   }
+  MM_HIP(hipGraphLaunch(it->second.second, e->stream));
   return 0;
 }
 

@@ -1245,14 +1245,18 @@ __global__ __launch_bounds__(256) void k_inner_time(EngPtrs P)
 // ---------------------------------------------------------------------------
 // launchers (explicit instantiations for the supported load-case counts)
 
+// nkern launches of the cycle A (0), B (1), R (2), starting at `first`.
 template <typename TS, int K, bool DICT>
-static int launch_steps(const EngPtrs &P, const SolveParams &sp, int grid, int nsteps, hipStream_t s)
+static int launch_steps(const EngPtrs &P, const SolveParams &sp, int grid, int first, int nkern, hipStream_t s)
 {
   if (grid % kXcd) return MM_ERR_ARG;  // the XCD-aware slab order needs whole groups of 8
-  for (int i = 0; i < nsteps; i++) {
-    k_step_a<TS, K, DICT><<<grid, P.SN, 0, s>>>(P, sp);
-    k_step_b<TS, K><<<grid, P.SN, 0, s>>>(P, sp);
-    k_step_r<TS, K><<<grid, P.SN, 0, s>>>(P, sp);
+  if (first < 0 || first > 2 || nkern < 0) return MM_ERR_ARG;
+  for (int i = first; i < first + nkern; i++) {
+    switch (i % 3) {
+      case 0: k_step_a<TS, K, DICT><<<grid, P.SN, 0, s>>>(P, sp); break;
+      case 1: k_step_b<TS, K><<<grid, P.SN, 0, s>>>(P, sp); break;
+      default: k_step_r<TS, K><<<grid, P.SN, 0, s>>>(P, sp); break;
+    }
   }
   MM_HIP(hipGetLastError());
   return 0;
@@ -1301,14 +1305,14 @@ static int launch_time(const EngPtrs &P, int grid, int what, hipStream_t s)
 bool eng_supported_rhs(int K) { return K == 1 || K == 6 || K == 7; }
 
 int eng_launch_steps(bool fp64, int K, bool dict, const EngPtrs &P, const SolveParams &sp, int grid,
-                     int nsteps, hipStream_t s)
+                     int first, int nkern, hipStream_t s)
 {
   if (dict) {
-#define CALL_STEPS(T, KK) launch_steps<T, KK, true>(P, sp, grid, nsteps, s)
+#define CALL_STEPS(T, KK) launch_steps<T, KK, true>(P, sp, grid, first, nkern, s)
     MM_DISPATCH(fp64, K, CALL_STEPS);
 #undef CALL_STEPS
   }
-#define CALL_STEPS(T, KK) launch_steps<T, KK, false>(P, sp, grid, nsteps, s)
+#define CALL_STEPS(T, KK) launch_steps<T, KK, false>(P, sp, grid, first, nkern, s)
   MM_DISPATCH(fp64, K, CALL_STEPS);
 #undef CALL_STEPS
 }

@@ -173,8 +173,9 @@ enum AsmMode : int { ASM_FULL = 0, ASM_X_ONLY = 1, ASM_DICT = 2 };
 // launchers (engine_kernels.hip)
 int upload_slot_tables(hipStream_t s);
 bool eng_supported_rhs(int K);
+// nkern solver launches of the cycle A (0) -> B (1) -> R (2), from `first`
 int eng_launch_steps(bool fp64, int K, bool dict, const EngPtrs &P, const SolveParams &sp, int grid,
-                     int nsteps, hipStream_t s);
+                     int first, int nkern, hipStream_t s);
 int eng_launch_dict(bool fp64, int n, const int *slots, const double *ke_tab, void *dict, hipStream_t s);
 int eng_launch_assemble(bool fp64, int K, const EngPtrs &P, int nblk, const int *slab_list, int mode,
                         hipStream_t s);

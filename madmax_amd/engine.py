@@ -85,7 +85,8 @@ class Engine:
     def __init__(self, nrhs=6, precision=_lib.MM_PREC_MIXED, hbm_budget=0, max_rves=0,
                  slab_nodes=0, rtol=1e-10, abs_tol=0.0, inner_rtol=1e-5, max_inner_its=20000,
                  max_sweeps=8, steps_per_launch=16, use_graph=True, sync_every_step=False,
-                 device=0, ell_storage=_lib.MM_ELL_DICT, initial_guess=_lib.MM_GUESS_AFFINE):
+                 device=0, ell_storage=_lib.MM_ELL_DICT, initial_guess=_lib.MM_GUESS_AFFINE,
+                 solver_streams=2):
         L = lib()
         c = EngineConfig()
         check(L.mm_engine_config_default(C.byref(c)), "mm_engine_config_default")
@@ -105,6 +106,7 @@ class Engine:
         c.sync_every_step = int(bool(sync_every_step))
         c.ell_storage = int(ell_storage)
         c.initial_guess = int(initial_guess)
+        c.solver_streams = int(solver_streams)
         self.cfg = c
         self.nrhs = nrhs
         h = C.c_void_p()

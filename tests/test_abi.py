@@ -102,3 +102,7 @@ def test_argument_errors_without_gpu():
     cfg.nrhs = 5  # unsupported block width
     h = C.c_void_p()
     assert L.mm_engine_create(C.byref(cfg), C.byref(h)) == _lib.MM_ERR_ARG
+    L.mm_engine_config_default(C.byref(cfg))
+    assert cfg.solver_streams == 2
+    cfg.solver_streams = 3  # one or two stream groups only
+    assert L.mm_engine_create(C.byref(cfg), C.byref(h)) == _lib.MM_ERR_ARG

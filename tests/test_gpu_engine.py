@@ -78,6 +78,22 @@ def test_alone_vs_in_mix_identical():
     assert alone["pcg_its"] == mixed["pcg_its"]
 
 
+@pytest.mark.parametrize("use_graph", [True, False])
+def test_one_and_two_solver_streams_identical(use_graph):
+    """Two stream groups (RVEs split in slab order, group 1 forked after group
+    0's first kernel) give the same bits as one stream, RVE by RVE."""
+    specs = [configs.c4_spec(5, i) for i in range(7)] + [configs.c2_spec(2, 3, n=10)]
+    res = {}
+    for streams in (1, 2):
+        with Engine(nrhs=6, hbm_budget=2 * GB, max_rves=16, solver_streams=streams,
+                    use_graph=use_graph) as eng:
+            res[streams] = _solve_batch(eng, specs)
+    for a, b in zip(res[1], res[2]):
+        assert a["status"] == 0 and b["status"] == 0
+        assert np.array_equal(a["c_tangent"], b["c_tangent"])
+        assert a["pcg_its"] == b["pcg_its"]
+
+
 def test_fd_tangent_seven_loads():
     spec = rve_spec(6, [(0.5, 0.5, 0.3)])
     eps = np.array([1e-3, 2e-4, -1e-4, 0.0, 3e-4, 0.0])
