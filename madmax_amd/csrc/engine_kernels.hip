@@ -864,9 +864,23 @@ __device__ __forceinline__ void inner_a(const EngPtrs &P, const RveDev *R, int s
   if (n < R->nrow / 3) {
     TS w[3][K], zc[3][K];
     slab_node_spmv<TS, K, DICT>(P, R, slot, slab, tid, n, dc, w, zc);
-    TS *p = (TS *)P.p;
-    TS *q = (TS *)P.q;
+    TS *__restrict__ p = (TS *)P.p;
+    TS *__restrict__ q = (TS *)P.q;
     const long long nb = sb + 3 * tid;
+    // every load of the node's p and q rows is issued before the first store
+    // (p and q do not alias), so the update costs one memory round trip, not
+    // one per row
+    TS pv[3][K], qv[3][K];
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+      if ((skip >> k) & 1u) continue;
+      const long long i = nb + (long long)k * RS;
+#pragma unroll
+      for (int d = 0; d < 3; d++) {
+        pv[d][k] = p[i + d];
+        qv[d][k] = q[i + d];
+      }
+    }
 #pragma unroll
     for (int k = 0; k < K; k++) {
       if ((skip >> k) & 1u) continue;
@@ -874,8 +888,8 @@ __device__ __forceinline__ void inner_a(const EngPtrs &P, const RveDev *R, int s
       const long long i = nb + (long long)k * RS;
 #pragma unroll
       for (int d = 0; d < 3; d++) {
-        const TS pi = fma(b, p[i + d], zc[d][k]);
-        const TS qi = fma(b, q[i + d], w[d][k]);
+        const TS pi = fma(b, pv[d][k], zc[d][k]);
+        const TS qi = fma(b, qv[d][k], w[d][k]);
         p[i + d] = pi;
         q[i + d] = qi;
         v[k] = fma((double)pi, (double)qi, v[k]);
