@@ -1005,8 +1005,10 @@ __global__ __launch_bounds__(256) void k_step_b(EngPtrs P, SolveParams sp)
   const int npair = rows >> 1;
   const long long sb = (long long)slab * K * RS;
   const TS *dinv = (const TS *)P.dinv + (long long)slab * RS;
-  TS *dv = (TS *)P.d, *zv = (TS *)P.z;
-  const TS *p = (const TS *)P.p, *q = (const TS *)P.q;
+  TS *__restrict__ dv = (TS *)P.d;
+  TS *__restrict__ zv = (TS *)P.z;
+  const TS *__restrict__ p = (const TS *)P.p;
+  const TS *__restrict__ q = (const TS *)P.q;
   double vu[2 * K];
 #pragma unroll
   for (int k = 0; k < K; k++) {
@@ -1015,24 +1017,39 @@ __global__ __launch_bounds__(256) void k_step_b(EngPtrs P, SolveParams sp)
     if ((skip >> k) & 1u) continue;
     const TS a = (TS)s_alpha[k];
     const long long kb = sb + (long long)k * RS;
-    for (int g = tid; g < npair; g += blockDim.x) {
-      const long long i = kb + 2 * g;
-      const T2 dg = *reinterpret_cast<const T2 *>(dinv + 2 * g);
-      T2 di = *reinterpret_cast<const T2 *>(dv + i);
-      T2 zi = *reinterpret_cast<const T2 *>(zv + i);
-      const T2 pi = *reinterpret_cast<const T2 *>(p + i);
-      const T2 qi = *reinterpret_cast<const T2 *>(q + i);
-      di.x = fma(a, pi.x, di.x);
-      di.y = fma(a, pi.y, di.y);
-      zi.x = fma(-a, dg.x * qi.x, zi.x);
-      zi.y = fma(-a, dg.y * qi.y, zi.y);
-      *reinterpret_cast<T2 *>(dv + i) = di;
-      *reinterpret_cast<T2 *>(zv + i) = zi;
-      const TS rx = zi.x / dg.x, ry = zi.y / dg.y;  // r = z / dinv
-      vu[k] = fma((double)rx, (double)rx, vu[k]);
-      vu[k] = fma((double)ry, (double)ry, vu[k]);
-      vu[K + k] = fma((double)rx, (double)zi.x, vu[K + k]);
-      vu[K + k] = fma((double)ry, (double)zi.y, vu[K + k]);
+    // two pairs per trip, both pairs' loads issued before either's stores;
+    // the dots still take the pairs in order tid, tid + blockDim, ...
+    const int bs = blockDim.x;
+    for (int g = tid; g < npair; g += 2 * bs) {
+      const bool two = g + bs < npair;
+      T2 dg[2], di[2], zi[2], pi[2], qi[2];
+#pragma unroll
+      for (int h = 0; h < 2; h++) {
+        if (h == 1 && !two) break;
+        const int gg = g + h * bs;
+        const long long i = kb + 2 * gg;
+        dg[h] = *reinterpret_cast<const T2 *>(dinv + 2 * gg);
+        di[h] = *reinterpret_cast<const T2 *>(dv + i);
+        zi[h] = *reinterpret_cast<const T2 *>(zv + i);
+        pi[h] = *reinterpret_cast<const T2 *>(p + i);
+        qi[h] = *reinterpret_cast<const T2 *>(q + i);
+      }
+#pragma unroll
+      for (int h = 0; h < 2; h++) {
+        if (h == 1 && !two) break;
+        const long long i = kb + 2 * (g + h * bs);
+        di[h].x = fma(a, pi[h].x, di[h].x);
+        di[h].y = fma(a, pi[h].y, di[h].y);
+        zi[h].x = fma(-a, dg[h].x * qi[h].x, zi[h].x);
+        zi[h].y = fma(-a, dg[h].y * qi[h].y, zi[h].y);
+        *reinterpret_cast<T2 *>(dv + i) = di[h];
+        *reinterpret_cast<T2 *>(zv + i) = zi[h];
+        const TS rx = zi[h].x / dg[h].x, ry = zi[h].y / dg[h].y;  // r = z / dinv
+        vu[k] = fma((double)rx, (double)rx, vu[k]);
+        vu[k] = fma((double)ry, (double)ry, vu[k]);
+        vu[K + k] = fma((double)rx, (double)zi[h].x, vu[K + k]);
+        vu[K + k] = fma((double)ry, (double)zi[h].y, vu[K + k]);
+      }
     }
     if ((rows & 1) && tid == 0) {
       const long long i = kb + rows - 1;
