@@ -71,6 +71,35 @@ def measured_traffic(batch, n):
     return None
 
 
+def ell_api_spmv(mm, spec, reps=20):
+    """The reference ELL API's y = A x on the device mirror (FP64 values,
+    int32 column ids, 81 slots per row, bit-exact slot order) for the FP64
+    matrix of one RVE. Bytes per launch: N * 81 * (8 + 4) + 16 N (matrix,
+    x once, y)."""
+    import ctypes as C
+    rows = 3 * (spec.nex + 1) * (spec.ney + 1) * (spec.nez + 1)
+    with mm.Engine(nrhs=1, precision=mm.MM_PREC_FP64, hbm_budget=int(rows * 81 * 16 * 1.3) + (256 << 20),
+                   max_rves=2, slab_nodes=256, inner_rtol=1e-10, rtol=1e-10, max_sweeps=2) as eng:
+        eng.submit(1, spec, [np.array([1e-3, 0, 0, 0, 0, 0])], mm.MM_TANGENT_NONE, warm_key=7)
+        eng.drain()
+        eng.poll()
+        cols, vals = eng.download_ell(7)
+    e = mm.Ell.from_arrays(cols, vals)
+    del cols, vals
+    h = C.c_void_p()
+    L = mm.lib()
+    mm._lib.check(L.ell_device_create(C.byref(e.m), 0, C.byref(h)), "ell_device_create")
+    ms = C.c_double()
+    try:
+        mm._lib.check(L.ell_device_time_mvp(h, reps, C.byref(ms)), "ell_device_time_mvp")
+    finally:
+        L.ell_device_destroy(h)
+        e.free()
+    nb = rows * 81 * 12 + 16 * rows
+    gbs = nb / (ms.value * 1e-3) / 1e9
+    return dict(ms=ms.value, bytes=nb, gbs=gbs, frac=gbs / HBM_PEAK_GBS, rows=rows)
+
+
 def workload_c3(args, mm, configs):
     """One 100^3 RVE, one random strain."""
     spec = configs.c3_spec()
@@ -100,6 +129,9 @@ def workload_c3(args, mm, configs):
                          step_gbs=sb / (sms * 1e-3) / 1e9)
         log("c3 %s: %.2f s, %d its, %d sweeps, SpMV %.3f ms -> %.0f GB/s"
             % (name, dt, r["pcg_its"], r["sweeps"], ms, out[name]["spmv_gbs"]))
+    out["ell_api_spmv"] = ell_api_spmv(mm, spec)
+    log("c3 reference-layout FP64 SpMV (ell_device_mvp): %.3f ms -> %.0f GB/s"
+        % (out["ell_api_spmv"]["ms"], out["ell_api_spmv"]["gbs"]))
     a, b = res["mixed_dict"]["stress"][0], res["fp64_dict"]["stress"][0]
     out["mixed_vs_fp64_stress_rel"] = float(np.max(np.abs(a - b)) / np.max(np.abs(b)))
     out["full_equals_dict"] = bool(np.array_equal(res["mixed_full"]["stress"][0], a))
@@ -328,6 +360,12 @@ def main():
         acc = max(errs)
         log("cpu oracle: %.3f tangents/s, max tangent err %.2e" % (cpu["value"], acc))
 
+    # the reference ELL API's SpMV (FP64, row-major slot order) on one RVE of
+    # the batch; its 87 MB matrix fits in the last-level cache
+    ell_api = ell_api_spmv(mm, specs[0])
+    ell_api["note"] = "one 30^3 RVE, FP64 + int32, 81 slots/row; matrix (87 MB) cache-resident"
+    log("reference-layout FP64 SpMV (ell_device_mvp): %.3f ms -> %.0f GB/s" % (ell_api["ms"], ell_api["gbs"]))
+
     # serving sample (FE^2 stream, mixed RVE sizes, Poisson arrivals)
     serving = None
     if args.serve > 0:
@@ -381,7 +419,7 @@ def main():
                      "copy_gbs_measured": copy_gbs},
         "spmv": {"dictionary": {"ms": spmv_ms, "bytes": spmv_bytes, "gbs": spmv_gbs,
                                 "frac": spmv_gbs / HBM_PEAK_GBS},
-                 "full_ell": full},
+                 "full_ell": full, "ell_api_fp64": ell_api},
         "cpu_baseline": cpu,
         "plain_fp64_gpu": plain,
         "speedup_vs_plain": (value / plain["value"]) if plain else None,

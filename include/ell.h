@@ -87,7 +87,9 @@ int ell_free(ell_matrix *m);
 int ell_solve_pcg(ell_solver *solver, ell_matrix *m, double *b, double *x);
 
 /* Device-resident mirror: slabs of slab_rows rows (a multiple of 64, 0 picks
- * the default 256), column-major inside a slab. */
+ * the default 256), column-major inside a slab.  Every column id must be -1
+ * or in [0, ncol) (what ell_add_val stores); create and update return 1
+ * otherwise, since the GPU would read x at that index. */
 typedef struct ell_device_ ell_device;
 int ell_device_create(const ell_matrix *m, int slab_rows, ell_device **d);
 /* Re-uploads cols/vals of a matrix with the same shape. */
@@ -96,6 +98,10 @@ int ell_device_update(ell_device *d, const ell_matrix *m);
 int ell_device_download(const ell_device *d, int *cols, double *vals);
 /* y = A x with host vectors (x, y of length nrow / ncol). */
 int ell_device_mvp(ell_device *d, const double *x, double *y);
+/* Average duration in ms of `reps` back-to-back device SpMV kernels on the
+ * resident vectors (HIP events on the mirror's stream; no host transfer).
+ * No counterpart in the reference: a measurement hook. */
+int ell_device_time_mvp(ell_device *d, int reps, double *ms);
 int ell_device_solve_pcg(ell_device *d, ell_solver *solver, const double *b, double *x);
 int ell_device_solve_jacobi(ell_device *d, ell_solver *solver, const double *b, double *x);
 int ell_device_destroy(ell_device *d);

@@ -121,6 +121,7 @@ SIGNATURES = {
     "ell_device_update": (C.c_int, [C.c_void_p, C.POINTER(EllMatrix)]),
     "ell_device_download": (C.c_int, [C.c_void_p, ip, dp]),
     "ell_device_mvp": (C.c_int, [C.c_void_p, dp, dp]),
+    "ell_device_time_mvp": (C.c_int, [C.c_void_p, C.c_int, dp]),
     "ell_device_solve_pcg": (C.c_int, [C.c_void_p, C.POINTER(EllSolver), dp, dp]),
     "ell_device_solve_jacobi": (C.c_int, [C.c_void_p, C.POINTER(EllSolver), dp, dp]),
     "ell_device_destroy": (C.c_int, [C.c_void_p]),

@@ -14,6 +14,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <vector>
@@ -39,16 +40,41 @@ struct IterState {
 // boundaries order the two, so no workgroup fences for another.
 enum RedMode : int { RED_JAC = 0, RED_INIT = 1, RED_PQ = 2, RED_RR = 3 };
 
+// Row `lane` of the slab times x: ascending slots, stop at the first -1,
+// separate multiply and add.  Slots are fetched kU at a time (column ids and
+// values in one round trip, the x entries in a second) and then accumulated
+// in slot order, so the loads of a batch overlap instead of waiting on each
+// other.  Column ids are -1 or in [0, ncol) (checked on upload), so the
+// entries fetched past a row's first -1 are in bounds and are never added.
+constexpr int kU = 8;
 __device__ inline double row_dot_exact(const int *__restrict__ cols, const double *__restrict__ vals,
                                        long long slab_base, int S, int nnz, int lane,
                                        const double *__restrict__ x)
 {
 #pragma clang fp contract(off)
   double acc = 0.0;
-  for (int j = 0; j < nnz; j++) {
-    const int c = cols[slab_base + (long long)j * S + lane];
-    if (c == -1) break;
-    acc = acc + vals[slab_base + (long long)j * S + lane] * x[c];
+  for (int j0 = 0; j0 < nnz; j0 += kU) {
+    int c[kU];
+    double v[kU], xv[kU];
+#pragma unroll
+    for (int u = 0; u < kU; u++) {
+      const bool in = j0 + u < nnz;
+      const long long e = slab_base + (long long)(j0 + u) * S + lane;
+      c[u] = in ? cols[e] : -1;
+      v[u] = in ? vals[e] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < kU; u++) xv[u] = (c[u] >= 0) ? x[c[u]] : 0.0;
+    bool stop = false;
+#pragma unroll
+    for (int u = 0; u < kU; u++) {
+      if (c[u] == -1) {
+        stop = true;
+        break;
+      }
+      acc = acc + v[u] * xv[u];
+    }
+    if (stop) break;
   }
   return acc;
 }
@@ -276,6 +302,21 @@ static void host_diag_inv(const ell_matrix *m, std::vector<double> &dinv)
   }
 }
 
+// The kernels read x[col] for every column id, so each must be -1 or a
+// column of the matrix (ell_add_val never stores anything else).
+static int check_cols(const ell_matrix *m)
+{
+  const size_t n = (size_t)m->nrow * m->nnz;
+  for (size_t e = 0; e < n; e++) {
+    const int c = m->cols[e];
+    if (c < -1 || c >= m->ncol) {
+      fprintf(stderr, "madmax_amd: ell column id %d at slot %zu outside [-1, %d)\n", c, e, m->ncol);
+      return 1;
+    }
+  }
+  return 0;
+}
+
 static int upload(ell_device *d, const ell_matrix *m)
 {
   const size_t slots = (size_t)d->nslab * d->S * d->nnz;
@@ -305,6 +346,7 @@ extern "C" int ell_device_create(const ell_matrix *m, int slab_rows, ell_device 
   if (m->nrow < 0 || m->nnz < 0 || m->ncol < 0) return 2;
   int S = slab_rows > 0 ? slab_rows : 256;
   if (S % 64 != 0 || S > 1024) return 1;
+  if (check_cols(m)) return 1;
   ell_device *d = (ell_device *)calloc(1, sizeof(ell_device));
   if (!d) return 2;
   d->nrow = m->nrow;
@@ -354,6 +396,7 @@ fail:
 extern "C" int ell_device_update(ell_device *d, const ell_matrix *m)
 {
   if (!d || !m || m->nrow != d->nrow || m->nnz != d->nnz || m->ncol != d->ncol) return 1;
+  if (check_cols(m)) return 1;
   return upload(d, m);
 }
 
@@ -374,6 +417,32 @@ extern "C" int ell_device_download(const ell_device *d, int *cols, double *vals)
     }
   }
   return 0;
+}
+
+extern "C" int ell_device_time_mvp(ell_device *d, int reps, double *ms)
+{
+  if (!d || !ms || reps < 1) return 1;
+  *ms = 0.0;
+  if (d->nrow == 0) return 0;
+  hipEvent_t a = nullptr, b = nullptr;
+  MM_HIP(hipEventCreate(&a));
+  if (hipEventCreate(&b) != hipSuccess) {
+    (void)hipEventDestroy(a);
+    return MM_ERR_HIP;
+  }
+  int rc = 0;
+  k_spmv64_exact<<<d->nslab, d->S, 0, d->stream>>>(d->cols, d->vals, d->nrow, d->nnz, d->S, d->x, d->q);
+  float t = 0.f;
+  if (hipEventRecord(a, d->stream) != hipSuccess) rc = MM_ERR_HIP;
+  for (int i = 0; i < reps && !rc; i++)
+    k_spmv64_exact<<<d->nslab, d->S, 0, d->stream>>>(d->cols, d->vals, d->nrow, d->nnz, d->S, d->x, d->q);
+  if (!rc && (hipGetLastError() != hipSuccess || hipEventRecord(b, d->stream) != hipSuccess ||
+              hipEventSynchronize(b) != hipSuccess || hipEventElapsedTime(&t, a, b) != hipSuccess))
+    rc = MM_ERR_HIP;
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  if (!rc) *ms = (double)t / reps;
+  return rc;
 }
 
 extern "C" int ell_device_mvp(ell_device *d, const double *x, double *y)

@@ -106,3 +106,8 @@ def test_argument_errors_without_gpu():
     assert cfg.solver_streams == 2
     cfg.solver_streams = 3  # one or two stream groups only
     assert L.mm_engine_create(C.byref(cfg), C.byref(h)) == _lib.MM_ERR_ARG
+    # a column id outside [-1, ncol) is refused before anything reaches the GPU
+    e = Ell.from_arrays(np.array([[0, 3]], dtype=np.int32), np.array([[1.0, 2.0]]), ncol=2)
+    assert L.ell_device_create(C.byref(e.m), 0, C.byref(h)) == 1
+    e = Ell.from_arrays(np.array([[0, -2]], dtype=np.int32), np.array([[1.0, 2.0]]), ncol=2)
+    assert L.ell_device_create(C.byref(e.m), 0, C.byref(h)) == 1

@@ -30,7 +30,8 @@ def test_ell_mvp_bitexact_c1():
     assert same_bits(y, y_ref)
 
 
-@pytest.mark.parametrize("nrow,ncol,nnz", [(1, 1, 1), (63, 70, 5), (300, 300, 9), (1000, 517, 13)])
+@pytest.mark.parametrize("nrow,ncol,nnz", [(1, 1, 1), (63, 70, 5), (300, 300, 9), (1000, 517, 13),
+                                            (257, 300, 16), (129, 1000, 81)])
 def test_ell_mvp_bitexact_random_with_gaps(nrow, ncol, nnz):
     """Random rows (including empty rows and a -1 hole before live slots,
     which the reference's break skips) and non-multiple-of-slab sizes."""
@@ -42,13 +43,26 @@ def test_ell_mvp_bitexact_random_with_gaps(nrow, ncol, nnz):
         cols[i, :k] = rng.choice(ncol, size=k, replace=(k > ncol))
         vals[i, :k] = rng.standard_normal(k)
         if k >= 3 and i % 5 == 0:
-            cols[i, 1] = -1  # everything after the first -1 is ignored
+            cols[i, rng.integers(1, k)] = -1  # everything after the first -1 is ignored
     x = rng.standard_normal(ncol)
     ref = po.OrEll(nrow, ncol, nnz, po._p(cols, po.ip), po._p(vals))
     y_ref = np.zeros(nrow)
     po.lib().or_ell_mvp(po.C.byref(ref), po._p(x), po._p(y_ref))
     y = Ell.from_arrays(cols, vals, ncol=ncol).mvp(x)
     assert same_bits(y, y_ref)
+
+
+def test_ell_device_time_mvp_runs():
+    import ctypes as C
+    _, cols, vals, _ = _c1_system()
+    e = Ell.from_arrays(cols, vals)
+    h = C.c_void_p()
+    assert lib().ell_device_create(C.byref(e.m), 0, C.byref(h)) == 0
+    ms = C.c_double()
+    assert lib().ell_device_time_mvp(h, 3, C.byref(ms)) == 0
+    assert ms.value > 0
+    assert lib().ell_device_time_mvp(h, 0, C.byref(ms)) == 1
+    assert lib().ell_device_destroy(h) == 0
 
 
 @pytest.mark.parametrize("slab", [64, 128, 256, 512])
