@@ -12,6 +12,8 @@ The JSON line also carries:
   spmv          the ELL SpMV alone: dictionary storage and the full
                 per-row ELL (the reference's layout) on the same batch
   cpu_baseline  the FP64 oracle (one core, one RVE at a time) on a sample
+  cpu_baseline_threads  the same oracle on P cores (P = usable cores, at
+                most 16), one RVE per thread
   plain_fp64    the plain FP64 single-RVE GPU path (one load case at a time)
   accuracy      max normwise relative tangent error vs the FP64 oracle
   serving       p50 / p99 latency of a short FE^2 request stream
@@ -338,6 +340,7 @@ def main():
 
     # CPU baseline: the FP64 oracle, one core, one RVE at a time (bounded sample)
     cpu = None
+    cpu_par = None
     acc = None
     if not args.no_cpu:
         from oracle import pyoracle as po
@@ -359,6 +362,26 @@ def main():
                           "+ averaging), oracle/oracle.c gcc -O3, one thread" % (args.cpu_sample, n))
         acc = max(errs)
         log("cpu oracle: %.3f tangents/s, max tangent err %.2e" % (cpu["value"], acc))
+
+        # the same oracle on P cores, one RVE per thread (the C calls release
+        # the GIL); P = the usable cores, at most 16
+        from concurrent.futures import ThreadPoolExecutor
+        P = max(1, min(16, len(os.sched_getaffinity(0))))
+
+        def one(j):
+            s = specs[j % len(specs)]
+            m = po.mesh(3, n, n, n)
+            ro = po.Rve(3, n, po.phase_map(m, mm.spec_fibers(s)), [s.young[0], s.young[1]],
+                        [s.poisson[0], s.poisson[1]])
+            ro.c_tangent_unit(1e-3, 1e-10)
+
+        t = time.perf_counter()
+        with ThreadPoolExecutor(P) as ex:
+            list(ex.map(one, range(P)))
+        tp_ = time.perf_counter() - t
+        cpu_par = dict(value=P / tp_, unit="RVE tangents/s", cores=P, kind="port",
+                       sample="%d seeded %d^3 RVE tangents, one per thread, same oracle" % (P, n))
+        log("cpu oracle, %d threads: %.3f tangents/s" % (P, cpu_par["value"]))
 
     # the reference ELL API's SpMV (FP64, row-major slot order) on one RVE of
     # the batch; its 87 MB matrix fits in the last-level cache
@@ -421,6 +444,7 @@ def main():
                                 "frac": spmv_gbs / HBM_PEAK_GBS},
                  "full_ell": full, "ell_api_fp64": ell_api},
         "cpu_baseline": cpu,
+        "cpu_baseline_threads": cpu_par,
         "plain_fp64_gpu": plain,
         "speedup_vs_plain": (value / plain["value"]) if plain else None,
         "accuracy": {"max_rel_err_tangent": acc, "all_converged": status_ok,
