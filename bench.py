@@ -219,8 +219,8 @@ def main():
     ap.add_argument("--count", type=int, default=0, help="c5 requests per pass")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py: this build measures one GPU; multi-GPU runs are not part of it "
-              "(RVEs shard across GPUs without communication, see DESIGN.md)", file=sys.stderr)
+        print("bench.py: measures one GPU; multi-GPU runs are not part of it "
+              "(RVEs shard across GPUs without communication)", file=sys.stderr)
         sys.exit(2)
 
     import madmax_amd as mm

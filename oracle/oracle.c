@@ -3,8 +3,8 @@
  * path, written from the reference's behaviour, not copied from it.
  *
  * TEST INFRASTRUCTURE ONLY.  Loaded by tests/, by __graft_entry__.smoke() and
- * by the CPU-baseline leg of bench.py as the checker; the product library
- * never uses it.
+ * by the CPU-baseline leg of bench.py, where its results are what the GPU
+ * results are compared with; the product library never uses it.
  *
  * Reference behaviours reproduced on purpose (they fix the data layout):
  *   - ELL first-touch column order and tail-only -1 padding (src/ell.c:46-84);

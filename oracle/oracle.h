@@ -1,8 +1,9 @@
 /*
  * oracle.h -- FP64 CPU restatement of the sputnik micro (RVE) homogenization
  * path.  TEST INFRASTRUCTURE ONLY: the parity tests, the smoke check and the
- * CPU-baseline leg of the benchmark load it as the checker.  The product
- * library (madmax_amd) never links, loads or calls anything in this folder.
+ * CPU-baseline leg of the benchmark load it as the reference they compare
+ * with.  The product library (madmax_amd) never links, loads or calls
+ * anything in this folder.
  *
  * Every function cites the reference file:line it follows (paths relative to
  * the reference root).  The reference implements only 2D for the structured
