@@ -60,8 +60,8 @@ __device__ inline double row_dot_exact(const int *__restrict__ cols, const doubl
     for (int u = 0; u < kU; u++) {
       const bool in = j0 + u < nnz;
       const long long e = slab_base + (long long)(j0 + u) * S + lane;
-      c[u] = in ? cols[e] : -1;
-      v[u] = in ? vals[e] : 0.0;
+      c[u] = in ? __builtin_nontemporal_load(&cols[e]) : -1;
+      v[u] = in ? __builtin_nontemporal_load(&vals[e]) : 0.0;
     }
 #pragma unroll
     for (int u = 0; u < kU; u++) xv[u] = (c[u] >= 0) ? x[c[u]] : 0.0;
