@@ -206,7 +206,10 @@ typedef struct mm_engine mm_engine;
 int mm_engine_config_default(mm_engine_config *cfg);
 int mm_engine_create(const mm_engine_config *cfg, mm_engine **eng);
 int mm_engine_destroy(mm_engine *eng);
-/* Queues a request (thread-safe); returns MM_REJECTED if it can never fit. */
+/* Queues a request (thread-safe); returns MM_REJECTED if it can never fit:
+ * its slabs or elements exceed the arena, or one solver vector of it (nrhs x
+ * 3 x nodes rounded up to slabs, times 4 or 8 bytes) reaches 2^31 bytes, the
+ * reach of the SpMV's 32-bit offsets (about 12.8 M nodes at 7 FP64 loads). */
 int mm_engine_submit(mm_engine *eng, const mm_request *req);
 /* One scheduling round: admit queued requests that fit, run
  * steps_per_launch PCG steps for every resident RVE, retire finished ones. */

@@ -6,6 +6,7 @@
 #include <algorithm>
 #include <atomic>
 #include <chrono>
+#include <climits>
 #include <cmath>
 #include <condition_variable>
 #include <cstdlib>
@@ -512,6 +513,9 @@ extern "C" int mm_engine_submit(mm_engine *e, const mm_request *req)
   long long nslab, nelm;
   footprint(e, req->rve, &nrow, &nslab, &nelm);
   if (nslab > e->nslab_cap || nelm > e->elem_cap) return MM_REJECTED;
+  // the SpMV addresses an RVE's run of a solver vector with 32-bit byte
+  // offsets (buffer loads), so one run must stay below 2^31 bytes
+  if ((long long)nslab * e->K * e->rows_per_slab() * (long long)e->tsz > INT_MAX) return MM_REJECTED;
   {
     std::lock_guard<std::mutex> lk(e->qmu);
     e->queue.push_back(Queued{*req, e->now()});

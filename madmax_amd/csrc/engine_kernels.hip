@@ -505,6 +505,31 @@ __device__ __forceinline__ T uniform_load(const T *p)
   return *(const __attribute__((address_space(4))) T *)p;
 }
 
+// Raw buffer view of one RVE's run of a solver vector: base at its first
+// slab, `bytes` long (below 2^31).  A load takes the lane's byte offset and a
+// wave-uniform byte offset (the load case's run) in a scalar register, so the
+// per-case address arithmetic of a 64-bit gather disappears.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t rve_buffer(const void *base, int bytes)
+{
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(base), (short)0, bytes, 0x00020000);
+}
+typedef float buf_f3 __attribute__((ext_vector_type(3)));
+__device__ __forceinline__ void buffer_load3(__amdgpu_buffer_rsrc_t r, unsigned off, unsigned soff,
+                                             float (&x)[3])
+{
+  const buf_f3 v = __builtin_amdgcn_raw_buffer_load_b96(r, (int)off, (int)soff, 0);
+  x[0] = v.x;
+  x[1] = v.y;
+  x[2] = v.z;
+}
+__device__ __forceinline__ void buffer_load3(__amdgpu_buffer_rsrc_t r, unsigned off, unsigned soff,
+                                             double (&x)[3])
+{
+#pragma unroll
+  for (int d = 0; d < 3; d++)
+    x[d] = __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, (int)(off + 8 * d), (int)soff, 0));
+}
+
 // Four values of a 16-byte aligned LDS row in one read.
 typedef float lds_f4 __attribute__((ext_vector_type(4)));
 typedef double lds_d2 __attribute__((ext_vector_type(2)));
@@ -536,8 +561,8 @@ enum DictSrc : int {
 // The 27-block product of an interior node from dictionary entry D, the
 // neighbours gathered in key order.
 template <typename TS, int K, int SRC>
-__device__ __forceinline__ void dict_rows(const EngPtrs &P, const TS *D, int first_slab, int n, int nx, int nxy,
-                                          const TS *__restrict__ z, TS (&w)[3][K], TS (&zc)[3][K])
+__device__ __forceinline__ void dict_rows(const EngPtrs &P, const TS *D, __amdgpu_buffer_rsrc_t zr, int n, int nx,
+                                          int nxy, TS (&w)[3][K], TS (&zc)[3][K])
 {
   const int RS = 3 * P.SN;
 #pragma unroll 3
@@ -556,12 +581,13 @@ __device__ __forceinline__ void dict_rows(const EngPtrs &P, const TS *D, int fir
 #pragma unroll
       for (int dd = 0; dd < 3; dd++) load_vec<TS, 4>(D + key * kDictBlk + dd * 4, v[dd]);
     }
-    const long long nb = node_vec(P, first_slab, m, K);
+    // byte offset of node m's load case 0 inside the RVE's run of slabs
+    const unsigned off =
+        (unsigned)(((m >> P.sn_shift) * K * RS + 3 * (m & (P.SN - 1))) * (int)sizeof(TS));
 #pragma unroll
     for (int k = 0; k < K; k++) {
       TS x[3];
-#pragma unroll
-      for (int dd = 0; dd < 3; dd++) x[dd] = z[nb + (long long)k * RS + dd];
+      buffer_load3(zr, off, (unsigned)(k * RS * (int)sizeof(TS)), x);
       if (key == 13)
 #pragma unroll
         for (int dd = 0; dd < 3; dd++) zc[dd][k] = x[dd];
@@ -655,15 +681,19 @@ __device__ __forceinline__ void node_spmv_dict(const EngPtrs &P, const RveDev *R
       zc[d][k] = (TS)0;
     }
   const TS *D0 = (const TS *)P.dict + (size_t)slot * kDictPat * kDictEntry;
+  // z of this RVE as a buffer; made here, before the branches below, so that
+  // the descriptor is plainly wave-uniform and stays in scalar registers
+  const int fs = __builtin_amdgcn_readfirstlane(R->first_slab), ns = __builtin_amdgcn_readfirstlane(R->nslab);
+  const __amdgpu_buffer_rsrc_t zr = rve_buffer(z + (long long)fs * K * RS, ns * K * RS * (int)sizeof(TS));
   const int pat = P.npat[(long long)slab * P.SN + tid];
   const int pat0 = __builtin_amdgcn_readfirstlane(pat);
   const int cs = dc.slot_of[pat];
   if (__all(pat == pat0))
-    dict_rows<TS, K, DICT_SCALAR>(P, D0 + (size_t)pat0 * kDictEntry, R->first_slab, n, nx, nxy, z, w, zc);
+    dict_rows<TS, K, DICT_SCALAR>(P, D0 + (size_t)pat0 * kDictEntry, zr, n, nx, nxy, w, zc);
   else if (__all(cs >= 0))
-    dict_rows<TS, K, DICT_LDS>(P, &dc.v[cs][0], R->first_slab, n, nx, nxy, z, w, zc);
+    dict_rows<TS, K, DICT_LDS>(P, &dc.v[cs][0], zr, n, nx, nxy, w, zc);
   else
-    dict_rows<TS, K, DICT_GLOBAL>(P, D0 + (size_t)pat * kDictEntry, R->first_slab, n, nx, nxy, z, w, zc);
+    dict_rows<TS, K, DICT_GLOBAL>(P, D0 + (size_t)pat * kDictEntry, zr, n, nx, nxy, w, zc);
 }
 
 // Either storage; dc is the workgroup's dictionary cache (dictionary storage
