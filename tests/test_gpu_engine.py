@@ -168,6 +168,20 @@ def test_admission_limits():
         assert all(r["status"] == 0 for r in out)
 
 
+def test_vector_run_limit_rejected():
+    """An RVE whose solver-vector run (nrhs x 3 x nodes x 8 bytes, 7 FP64
+    loads) reaches 2^31 bytes is refused even though the arena could hold its
+    slabs: the SpMV addresses a run with 32-bit offsets. The 12 GB arena holds
+    about 57,900 slabs (222 KB each at 7 FP64 loads); 235^3 elements need
+    51,345 slabs and 2.21 GB per run (rejected), 200^3 need 31,722 slabs and
+    1.36 GB per run (queued; nothing is solved here)."""
+    with Engine(nrhs=7, precision=MM_PREC_FP64, hbm_budget=12 * GB, max_rves=2,
+                slab_nodes=256) as eng:
+        loads = [np.eye(6)[i % 6] * 1e-3 for i in range(7)]
+        assert eng.submit(1, rve_spec(235), loads, MM_TANGENT_NONE) == _lib.MM_REJECTED
+        assert eng.submit(2, rve_spec(200), loads, MM_TANGENT_NONE) == 0
+
+
 def test_not_converged_status():
     with Engine(nrhs=6, hbm_budget=GB // 4, max_rves=4, max_inner_its=2, max_sweeps=1) as eng:
         assert eng.submit(1, rve_spec(8, [(0.5, 0.5, 0.3)]), unit_loads(), MM_TANGENT_UNIT) == 0
