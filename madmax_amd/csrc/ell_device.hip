@@ -102,7 +102,7 @@ __global__ __launch_bounds__(256) void k_reduce(const double *__restrict__ part,
     return;
   }
   double s[3];
-  group_sum<3>(part, nslab, 4, 0, s, lds);
+  group_sum_rows<3>(part, nslab, 4, s, lds);
   if (threadIdx.x != 0) return;
   switch (mode) {
     case RED_JAC:  // src/ell.c:160-180: err = ||A x - b||; stop, or update and count

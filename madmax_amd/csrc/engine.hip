@@ -232,6 +232,7 @@ struct mm_engine {
     P.p = p;
     P.q = q;
     P.part = part;
+    P.part_stride = nslab_cap;
     P.slab_rve = slab_rve;
     P.slab_node0 = slab_node0;
     P.rves = rves;

@@ -739,15 +739,22 @@ struct Pair<double> {
 };
 
 // Per-slab partial sums of the launch that produced them, summed over the
-// RVE's slabs in slab order (group_sum): every workgroup of the consuming
-// launch forms the same totals.  Regions of a slab's kPart doubles:
+// RVE's slabs in slab order (group_sum_cols): every workgroup of the consuming
+// launch forms the same totals.  A slab has kPart partial values, value i
+// at part[i * part_stride + slab] (value-major, so that a sum over an RVE's
+// slabs reads contiguous memory).  Values:
 //   [0, K)      p.q            (A -> B)
 //   [K, 3K)     r.r, r.z       (B -> A)
 //   [3K, 7K)    residual norms (R -> A)
 //   [0, 12K)    averages       (R -> A), never pending with the others
-__device__ __forceinline__ double *slab_part(const EngPtrs &P, int slab)
+struct SlabPart {
+  double *p;
+  long long s;
+  __device__ __forceinline__ double &operator[](int i) const { return p[(size_t)i * s]; }
+};
+__device__ __forceinline__ SlabPart slab_part(const EngPtrs &P, int slab)
 {
-  return P.part + (size_t)slab * kPart;
+  return SlabPart{P.part + slab, P.part_stride};
 }
 
 // Start of a refinement sweep from the reduced residual norms s4 =
@@ -824,11 +831,11 @@ __device__ void finish_rve(const EngPtrs &P, int slot, const RveDev *R, SolveSta
                            const SolveParams &sp, double (&lds)[RedLds<K>::size])
 {
   __shared__ double tot[MM_MAX_RHS][12];
-  const double *p0 = slab_part(P, R->first_slab);
+  const double *p0 = slab_part(P, R->first_slab).p;
 #pragma unroll 1
   for (int k = 0; k < K; k++) {
     double s12[12];
-    group_sum<12>(p0, R->nslab, kPart, k * 12, s12, lds);
+    group_sum_cols<12>(p0, R->nslab, P.part_stride, k * 12, s12, lds);
     if (threadIdx.x == 0)
       for (int i = 0; i < 12; i++) tot[k][i] = s12[i];
   }
@@ -928,7 +935,7 @@ __device__ __forceinline__ void inner_a(const EngPtrs &P, const RveDev *R, int s
   }
   block_sum<K>(v, lds);
   if (tid == 0) {
-    double *part = slab_part(P, slab);
+    const SlabPart part = slab_part(P, slab);
     for (int k = 0; k < K; k++) part[k] = v[k];
   }
 }
@@ -957,7 +964,7 @@ __global__ __launch_bounds__(256) void k_step_a(EngPtrs P, SolveParams sp)
   const bool writer = slab == R->first_slab;
   const int nslab = R->nslab;
   const int phase0 = R->st[2].phase;
-  const double *p0 = slab_part(P, R->first_slab);
+  const double *p0 = slab_part(P, R->first_slab).p;
   if (phase0 == PH_AVGRED) {
     if (!writer) return;
     if (tid == 0) N = R->st[2];
@@ -972,11 +979,11 @@ __global__ __launch_bounds__(256) void k_step_a(EngPtrs P, SolveParams sp)
   }
   if (phase0 == PH_RESRED) {
     double s4[4 * K];
-    group_sum<4 * K>(p0, nslab, kPart, 3 * K, s4, lds);
+    group_sum_cols<4 * K>(p0, nslab, P.part_stride, 3 * K, s4, lds);
     if (tid == 0) start_sweep<K>(N, s4, R, sp);
   } else if (phase0 == PH_INNER) {
     double s2[2 * K];
-    group_sum<2 * K>(p0, nslab, kPart, K, s2, lds);
+    group_sum_cols<2 * K>(p0, nslab, P.part_stride, K, s2, lds);
     if (tid == 0) s_xupd = inner_check<K>(N, s2, sp) ? 1 : 0;
   }
   __syncthreads();
@@ -1025,7 +1032,7 @@ __global__ __launch_bounds__(256) void k_step_b(EngPtrs P, SolveParams sp)
   const unsigned skip = S->done_in;
   const int nslab = R->nslab;
   double s1[K];
-  group_sum<K>(slab_part(P, R->first_slab), nslab, kPart, 0, s1, lds);
+  group_sum_cols<K>(slab_part(P, R->first_slab).p, nslab, P.part_stride, 0, s1, lds);
   if (tid == 0)
     for (int k = 0; k < K; k++)
       s_alpha[k] = (((skip >> k) & 1u) || s1[k] == 0.0) ? 0.0 : S->rz[k] / s1[k];
@@ -1095,7 +1102,7 @@ __global__ __launch_bounds__(256) void k_step_b(EngPtrs P, SolveParams sp)
   }
   block_sum<2 * K>(vu, lds);
   if (tid == 0) {
-    double *part = slab_part(P, slab);
+    const SlabPart part = slab_part(P, slab);
     for (int i = 0; i < 2 * K; i++) part[K + i] = vu[i];
   }
 }
@@ -1191,7 +1198,7 @@ __device__ void res_slab(const EngPtrs &P, int slab, int slot, const RveDev *R, 
   }
   block_sum<4 * K>(v, lds);
   if (tid == 0) {
-    double *part = slab_part(P, slab);
+    const SlabPart part = slab_part(P, slab);
     for (int i = 0; i < 4 * K; i++) part[3 * K + i] = v[i];
   }
 }
@@ -1204,7 +1211,7 @@ __device__ void avg_slab(const EngPtrs &P, int slab, int slot, const RveDev *R,
   __shared__ double cm[kNmat * 36];
   const int tid = threadIdx.x;
   const int nslab = R->nslab;
-  double *part = slab_part(P, slab);
+  const SlabPart part = slab_part(P, slab);
   const int ls = slab - R->first_slab;
   const int nelm = R->nelm;
   const int epb = (nelm + nslab - 1) / nslab;

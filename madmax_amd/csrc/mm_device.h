@@ -33,20 +33,35 @@ __device__ inline void block_sum(double (&v)[NV], double (&lds)[L])
   __syncthreads();
 }
 
-// Run by every thread of a workgroup: sums NV consecutive partials (offset
-// off) over nslab partial rows (row stride `stride`) written by an earlier
-// launch (visible at kernel start, so plain cached loads), lane-strided in
-// slab order, then block_sum.  The total is valid in thread 0; every
-// workgroup that calls it over the same partials obtains the same bits.
+// Run by every thread of a workgroup: sums partial values off .. off+NV-1
+// over nslab slabs written by an earlier launch (visible at kernel start, so
+// plain cached loads).  Value i of slab b is p0[(off + i) * vstride + b]:
+// each value's partials are contiguous, so a workgroup reads nslab * 8 bytes
+// per value, not one cache line per slab.  Lane-strided in slab order, then
+// block_sum; the total is valid in thread 0, and every workgroup that calls
+// it over the same partials obtains the same bits.
 template <int NV, int L>
-__device__ inline void group_sum(const double *p0, int nslab, int stride, int off, double (&out)[NV],
-                                 double (&lds)[L])
+__device__ inline void group_sum_cols(const double *p0, int nslab, long long vstride, int off, double (&out)[NV],
+                                      double (&lds)[L])
 {
 #pragma unroll
   for (int i = 0; i < NV; i++) out[i] = 0.0;
   for (int b = threadIdx.x; b < nslab; b += blockDim.x)
 #pragma unroll
-    for (int i = 0; i < NV; i++) out[i] += p0[(size_t)b * stride + off + i];
+    for (int i = 0; i < NV; i++) out[i] += p0[(size_t)(off + i) * vstride + b];
+  block_sum<NV>(out, lds);
+}
+
+// The same over row-major partials: value i of slab b at p0[b * stride + i].
+template <int NV, int L>
+__device__ inline void group_sum_rows(const double *p0, int nslab, int stride, double (&out)[NV],
+                                      double (&lds)[L])
+{
+#pragma unroll
+  for (int i = 0; i < NV; i++) out[i] = 0.0;
+  for (int b = threadIdx.x; b < nslab; b += blockDim.x)
+#pragma unroll
+    for (int i = 0; i < NV; i++) out[i] += p0[(size_t)b * stride + i];
   block_sum<NV>(out, lds);
 }
 

@@ -141,7 +141,8 @@ struct EngPtrs {
   const void *dinv;
   double *x64;
   void *d, *z, *p, *q;  // residual kept only as z = dinv r
-  double *part;
+  double *part;           // [value][slab] partial sums, value stride part_stride
+  long long part_stride;  // slabs of the arena
   const int *slab_rve, *slab_node0;
   RveDev *rves;
   RveOut *out;
