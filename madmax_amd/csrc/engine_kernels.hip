@@ -1065,11 +1065,18 @@ __global__ __launch_bounds__(256) void k_step_b(EngPtrs P, SolveParams sp)
         if (h == 1 && !two) break;
         const int gg = g + h * bs;
         const long long i = kb + 2 * gg;
+        // d, p and q are not read again before the next iteration has
+        // streamed a whole working set (larger than the last-level cache):
+        // non-temporal, so that they do not evict z, the dictionaries and
+        // the partials
         dg[h] = *reinterpret_cast<const T2 *>(dinv + 2 * gg);
-        di[h] = *reinterpret_cast<const T2 *>(dv + i);
+        di[h].x = __builtin_nontemporal_load(dv + i);
+        di[h].y = __builtin_nontemporal_load(dv + i + 1);
         zi[h] = *reinterpret_cast<const T2 *>(zv + i);
-        pi[h] = *reinterpret_cast<const T2 *>(p + i);
-        qi[h] = *reinterpret_cast<const T2 *>(q + i);
+        pi[h].x = __builtin_nontemporal_load(p + i);
+        pi[h].y = __builtin_nontemporal_load(p + i + 1);
+        qi[h].x = __builtin_nontemporal_load(q + i);
+        qi[h].y = __builtin_nontemporal_load(q + i + 1);
       }
 #pragma unroll
       for (int h = 0; h < 2; h++) {
@@ -1079,7 +1086,8 @@ __global__ __launch_bounds__(256) void k_step_b(EngPtrs P, SolveParams sp)
         di[h].y = fma(a, pi[h].y, di[h].y);
         zi[h].x = fma(-a, dg[h].x * qi[h].x, zi[h].x);
         zi[h].y = fma(-a, dg[h].y * qi[h].y, zi[h].y);
-        *reinterpret_cast<T2 *>(dv + i) = di[h];
+        __builtin_nontemporal_store(di[h].x, dv + i);
+        __builtin_nontemporal_store(di[h].y, dv + i + 1);
         *reinterpret_cast<T2 *>(zv + i) = zi[h];
         const TS rx = zi[h].x / dg[h].x, ry = zi[h].y / dg[h].y;  // r = z / dinv
         vu[k] = fma((double)rx, (double)rx, vu[k]);
