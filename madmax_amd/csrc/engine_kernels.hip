@@ -906,7 +906,8 @@ __device__ __forceinline__ void inner_a(const EngPtrs &P, const RveDev *R, int s
     const long long nb = sb + 3 * tid;
     // every load of the node's p and q rows is issued before the first store
     // (p and q do not alias), so the update costs one memory round trip, not
-    // one per row
+    // one per row; p and q are streamed non-temporally (next touched a whole
+    // working set later), which leaves the cache to z and the dictionaries
     TS pv[3][K], qv[3][K];
 #pragma unroll
     for (int k = 0; k < K; k++) {
@@ -914,8 +915,8 @@ __device__ __forceinline__ void inner_a(const EngPtrs &P, const RveDev *R, int s
       const long long i = nb + (long long)k * RS;
 #pragma unroll
       for (int d = 0; d < 3; d++) {
-        pv[d][k] = p[i + d];
-        qv[d][k] = q[i + d];
+        pv[d][k] = __builtin_nontemporal_load(p + i + d);
+        qv[d][k] = __builtin_nontemporal_load(q + i + d);
       }
     }
 #pragma unroll
@@ -927,8 +928,8 @@ __device__ __forceinline__ void inner_a(const EngPtrs &P, const RveDev *R, int s
       for (int d = 0; d < 3; d++) {
         const TS pi = fma(b, pv[d][k], zc[d][k]);
         const TS qi = fma(b, qv[d][k], w[d][k]);
-        p[i + d] = pi;
-        q[i + d] = qi;
+        __builtin_nontemporal_store(pi, p + i + d);
+        __builtin_nontemporal_store(qi, q + i + d);
         v[k] = fma((double)pi, (double)qi, v[k]);
       }
     }
