@@ -156,10 +156,10 @@ struct mm_engine {
   size_t tsz = 4;
   hipStream_t stream = nullptr;
   // solver_streams == 2: the resident RVEs are split into two groups of
-  // contiguous slabs, each stepped on its own stream (stream, stream2), so the
-  // gather-bound kernel A of one group runs beside the streaming kernel B of
-  // the other.  ev_fork / ev_join order stream2 after the admission work and
-  // the engine stream after stream2's steps.
+  // contiguous slabs, each stepped on its own stream (stream, stream2), so
+  // that one group's kernels fill the other's launch gaps and tails.
+  // ev_fork / ev_join order stream2 after the admission work and the engine
+  // stream after stream2's steps.
   hipStream_t stream2 = nullptr;
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   // arena
@@ -778,9 +778,10 @@ static int launch_grid(long long n)
 
 // Queues steps_per_launch solver steps (A, B, R each) for every RVE being
 // solved: group 0 on the engine stream; group 1, if any, on stream2, forked
-// after group 0's first kernel A so that the two chains start one kernel
-// apart (A of one group tends to run beside B of the other), and joined
-// before the engine stream's other work.
+// after group 0's first kernel A and joined before the engine stream's other
+// work.  The second chain fills the first one's launch gaps and kernel tails
+// (A's register use leaves no room for B's waves beside it, so the two do not
+// otherwise overlap).
 static int queue_steps(mm_engine *e, int grid0, int grid1)
 {
   EngPtrs P0 = e->ptrs(), P1 = P0;
