@@ -78,6 +78,25 @@ def test_alone_vs_in_mix_identical():
     assert alone["pcg_its"] == mixed["pcg_its"]
 
 
+def test_slab_shapes_agree():
+    """The same uneven batch with 64, 128 and 256 nodes per slab: every slab
+    shape meets the oracle bar, and the shapes agree closely with each other
+    (only the grouping of the partial sums differs)."""
+    specs = [configs.c4_spec(6, i) for i in range(6)]
+    specs = [s for s in specs if s.nex <= 16][:3] + [rve_spec((9, 7, 5), [(0.4, 0.55, 0.3)])]
+    refs = [_oracle_tangent(s) for s in specs]
+    out = {}
+    for sn in (64, 128, 256):
+        with Engine(nrhs=6, hbm_budget=GB, max_rves=16, slab_nodes=sn) as eng:
+            out[sn] = _solve_batch(eng, specs)
+        for s, r, c_ref in zip(specs, out[sn], refs):
+            assert r["status"] == 0
+            assert normwise_rel(r["c_tangent"], c_ref) <= TOL, (sn, s.nex)
+    for a, b, c in zip(out[64], out[128], out[256]):
+        assert normwise_rel(a["c_tangent"], c["c_tangent"]) <= 1e-12
+        assert normwise_rel(b["c_tangent"], c["c_tangent"]) <= 1e-12
+
+
 @pytest.mark.parametrize("use_graph", [True, False])
 def test_one_and_two_solver_streams_identical(use_graph):
     """Two stream groups (RVEs split in slab order, group 1 forked after group
