@@ -344,6 +344,12 @@ def main():
     acc = None
     if not args.no_cpu:
         from oracle import pyoracle as po
+        model = "unknown CPU"
+        try:
+            with open("/proc/cpuinfo") as f:
+                model = next((ln.split(":", 1)[1].strip() for ln in f if ln.startswith("model name")), model)
+        except OSError:
+            pass
         errs = []
         t = time.perf_counter()
         for j in range(args.cpu_sample):
@@ -359,7 +365,8 @@ def main():
         tc = time.perf_counter() - t
         cpu = dict(value=args.cpu_sample / tc, unit="RVE tangents/s", cores=1, kind="port",
                    sample="%d seeded %d^3 RVE tangents (assembly + 6 FP64 Jacobi-PCG solves to 1e-10 "
-                          "+ averaging), oracle/oracle.c gcc -O3, one thread" % (args.cpu_sample, n))
+                          "+ averaging), oracle/oracle.c gcc -O3, one thread" % (args.cpu_sample, n),
+                   cpu_model=model)
         acc = max(errs)
         log("cpu oracle: %.3f tangents/s, max tangent err %.2e" % (cpu["value"], acc))
 
@@ -380,7 +387,8 @@ def main():
             list(ex.map(one, range(P)))
         tp_ = time.perf_counter() - t
         cpu_par = dict(value=P / tp_, unit="RVE tangents/s", cores=P, kind="port",
-                       sample="%d seeded %d^3 RVE tangents, one per thread, same oracle" % (P, n))
+                       sample="%d seeded %d^3 RVE tangents, one per thread, same oracle" % (P, n),
+                       cpu_model=model)
         log("cpu oracle, %d threads: %.3f tangents/s" % (P, cpu_par["value"]))
 
     # the reference ELL API's SpMV (FP64, row-major slot order) on one RVE of
