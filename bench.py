@@ -6,33 +6,48 @@ six unit-strain load cases solved as a 6-RHS block by FP32 Jacobi-PCG inside
 FP64 iterative refinement, and their 6x6 tangents averaged in FP64.
 
 The JSON line also carries:
-  roofline      the solve's dominant kernel (the PCG step's SpMV + p/q update,
-                dictionary ELL storage), timed with HIP events on the engine
-                stream over a stand-alone replica of its work
-  spmv          the ELL SpMV alone: dictionary storage and the full
-                per-row ELL (the reference's layout) on the same batch
+  roofline      the solve's dominant kernel, kernel A (the PCG step's SpMV +
+                p/q update, dictionary ELL storage): the median duration of
+                its launches inside a solve of the batch (HIP events around
+                every launch, one stream, no graph), against the algorithmic
+                bytes of one launch; the stand-alone replica of its work is
+                reported beside it
+  spmv          the ELL SpMV alone: dictionary storage and the full per-row
+                ELL on the batch, and the HBM-resident figures of one 100^3
+                RVE (2.0 GB FP32 ELL, 3.05 GB FP64 reference-layout ELL)
+  c3            the 100^3 RVE: mixed solve time, iterations, sweeps, stress
+                error vs the FP64 oracle fixture
+  ceiling       hand-written streaming kernels (read-only and copy): the
+                HBM rate a kernel of that access pattern reaches on the box
   cpu_baseline  the FP64 oracle (one core, one RVE at a time) on a sample
-  cpu_baseline_threads  the same oracle on P cores (P = usable cores, at
-                most 16), one RVE per thread
-  plain_fp64    the plain FP64 single-RVE GPU path (one load case at a time)
-  accuracy      max normwise relative tangent error vs the FP64 oracle
+  cpu_baseline_threads  the same oracle on every usable core, one RVE per
+                thread
+  fp64_batched  the same batch in FP64 (FP64 PCG, same engine)
+  plain_fp64_gpu  the plain FP64 single-RVE path (one load case at a time,
+                zero start): with a host check after every PCG step, and with
+                graph-replayed steps
+  accuracy      max normwise relative tangent error vs the FP64 oracle over
+                every RVE the CPU legs solved
   serving       p50 / p99 latency of a short FE^2 request stream
 
 --kernels-only admits and solves the batch once, then runs the timing
-replicas and the HBM copy probe (the workload of the counter runs).
+replicas and the streaming probes (the workload of the counter runs).
 
 --workload selects other seeded workloads, each printing its own JSON line:
   c3  one 100^3 RVE (3.09 M dofs, 4 fibers, random strain): solve time,
       PCG iterations, refinement sweeps, SpMV GB/s of both storages, and the
-      mixed-precision stress against an FP64 solve
+      mixed-precision stress against an FP64 solve and the oracle fixture
   c4  512 RVEs of uneven size (8^3..32^3), 6 load cases each: tangents/s
   c5  FE^2 serving stream (12^3/20^3/30^3 RVEs, strain + tangent per request,
-      Poisson arrivals), two Newton passes over the same gauss points, the
-      second warm-started from the resident RVEs: p50 / p99 latency
+      Poisson arrivals): a sweep of arrival rates (throughput, p50 / p99,
+      the saturation knee), then the full 32,768-request stream with a
+      warm-started second Newton pass and the stress error of a 64-request
+      sample against the oracle fixture
 """
 import argparse
 import json
 import os
+import subprocess
 import sys
 import time
 
@@ -43,11 +58,17 @@ import numpy as np  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "pmc_k_inner_time_128x30cube.md")
+GOLD = os.path.join(ROOT, "tests", "golden")
 METRIC = "RVE tangents/sec at 30³ hex8 (rel err≤1e-8); p99 latency; SpMV HBM GB/s"
 
 
 def log(msg):
     print("[bench] " + msg, file=sys.stderr, flush=True)
+
+
+def rel(a, b):
+    a, b = np.asarray(a, dtype=np.float64), np.asarray(b, dtype=np.float64)
+    return float(np.max(np.abs(a - b)) / np.max(np.abs(b)))
 
 
 def run_batch(eng, specs, mm, warm=False):
@@ -63,7 +84,7 @@ def run_batch(eng, specs, mm, warm=False):
 
 def measured_traffic(batch, n):
     """HBM bytes per launch of the step kernel from the committed counter runs
-    (FETCH_SIZE + WRITE_SIZE, calibrated on the copy kernel), if they were
+    (FETCH_SIZE + WRITE_SIZE, calibrated on a copy kernel), if they were
     taken on this workload."""
     if batch != 128 or n != 30 or not os.path.exists(TRAFFIC_FILE):
         return None
@@ -73,15 +94,28 @@ def measured_traffic(batch, n):
     return None
 
 
+def stream_ceilings(mm, nbytes=4 << 30, reps=10):
+    """The hand-written read-only and copy kernels over nbytes: GB/s."""
+    import ctypes as C
+    out = {}
+    for name, mode, factor in (("read", 1, 1), ("copy", 0, 2)):
+        ms = C.c_double()
+        rc = mm.lib().mm_hbm_stream_probe(0, nbytes, reps, mode, C.byref(ms))
+        out[name + "_gbs"] = (factor * nbytes / (ms.value * 1e-3) / 1e9) if rc == 0 and ms.value > 0 else None
+    out["bytes"] = nbytes
+    return out
+
+
 def ell_api_spmv(mm, spec, reps=20):
     """The reference ELL API's y = A x on the device mirror (FP64 values,
     int32 column ids, 81 slots per row, bit-exact slot order) for the FP64
-    matrix of one RVE. Bytes per launch: N * 81 * (8 + 4) + 16 N (matrix,
-    x once, y)."""
+    matrix of one RVE, assembled on the GPU by an FP64 engine (no solve).
+    Bytes per launch: N * 81 * (8 + 4) + 16 N (matrix, x once, y)."""
     import ctypes as C
     rows = 3 * (spec.nex + 1) * (spec.ney + 1) * (spec.nez + 1)
     with mm.Engine(nrhs=1, precision=mm.MM_PREC_FP64, hbm_budget=int(rows * 81 * 16 * 1.3) + (256 << 20),
-                   max_rves=2, slab_nodes=256, inner_rtol=1e-10, rtol=1e-10, max_sweeps=2) as eng:
+                   max_rves=2, slab_nodes=256, inner_rtol=1e-10, rtol=1e-10, max_sweeps=1,
+                   max_inner_its=1) as eng:
         eng.submit(1, spec, [np.array([1e-3, 0, 0, 0, 0, 0])], mm.MM_TANGENT_NONE, warm_key=7)
         eng.drain()
         eng.poll()
@@ -102,16 +136,21 @@ def ell_api_spmv(mm, spec, reps=20):
     return dict(ms=ms.value, bytes=nb, gbs=gbs, frac=gbs / HBM_PEAK_GBS, rows=rows)
 
 
-def workload_c3(args, mm, configs):
-    """One 100^3 RVE, one random strain."""
-    spec = configs.c3_spec()
-    eps = configs.c3_strain()
+def c3_fixture():
+    path = os.path.join(GOLD, "wl_c3.npz")
+    return np.load(path) if os.path.exists(path) else None
+
+
+def c3_solves(mm, configs, names):
+    """Solves the 100^3 RVE with the named engines; returns results, the
+    stand-alone SpMV / step-kernel timings and the solve times."""
+    spec, eps = configs.c3_spec(), configs.c3_strain()
     rows = 3 * 101 ** 3
-    out = {"workload": "c3: one 100^3 hex8 RVE (%d dofs), 4 fibers vf 0.4, strain ~ N(0, 1e-3)" % rows}
-    res = {}
-    for name, prec, storage in (("mixed_dict", mm.MM_PREC_MIXED, mm.MM_ELL_DICT),
-                                ("mixed_full", mm.MM_PREC_MIXED, mm.MM_ELL_FULL),
-                                ("fp64_dict", mm.MM_PREC_FP64, mm.MM_ELL_DICT)):
+    table = {"mixed_dict": (mm.MM_PREC_MIXED, mm.MM_ELL_DICT), "mixed_full": (mm.MM_PREC_MIXED, mm.MM_ELL_FULL),
+             "fp64_dict": (mm.MM_PREC_FP64, mm.MM_ELL_DICT)}
+    out = {}
+    for name in names:
+        prec, storage = table[name]
         budget = int(rows * (81 * 8 + 200) * 1.3) + (512 << 20)
         with mm.Engine(nrhs=1, precision=prec, hbm_budget=budget, max_rves=2, slab_nodes=256,
                        rtol=1e-10, inner_rtol=1e-5 if prec == mm.MM_PREC_MIXED else 1e-10,
@@ -124,19 +163,39 @@ def workload_c3(args, mm, configs):
             dt = time.perf_counter() - t
             ms, nb = eng.time_spmv(reps=20)
             sms, sb = eng.time_step(reps=20)
-        res[name] = r
-        out[name] = dict(seconds=dt, status=r["status"], pcg_its=r["pcg_its"], sweeps=r["sweeps"],
-                         residual_rel=r["residual_rel"], spmv_ms=ms, spmv_gbs=nb / (ms * 1e-3) / 1e9,
-                         spmv_frac=nb / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, step_ms=sms,
-                         step_gbs=sb / (sms * 1e-3) / 1e9)
-        log("c3 %s: %.2f s, %d its, %d sweeps, SpMV %.3f ms -> %.0f GB/s"
-            % (name, dt, r["pcg_its"], r["sweeps"], ms, out[name]["spmv_gbs"]))
-    out["ell_api_spmv"] = ell_api_spmv(mm, spec)
+        out[name] = dict(result=r, seconds=dt, spmv_ms=ms, spmv_bytes=nb, step_ms=sms, step_bytes=sb)
+        log("c3 %s: %.3f s, %d its, %d sweeps, SpMV %.3f ms -> %.0f GB/s"
+            % (name, dt, r["pcg_its"], r["sweeps"], ms, nb / (ms * 1e-3) / 1e9))
+    return out
+
+
+def c3_report(res, fix):
+    out = {}
+    for name, v in res.items():
+        r = v["result"]
+        gbs = v["spmv_bytes"] / (v["spmv_ms"] * 1e-3) / 1e9
+        out[name] = dict(seconds=v["seconds"], status=r["status"], pcg_its=r["pcg_its"], sweeps=r["sweeps"],
+                         residual_rel=r["residual_rel"], spmv_ms=v["spmv_ms"], spmv_bytes=v["spmv_bytes"],
+                         spmv_gbs=gbs, spmv_frac=gbs / HBM_PEAK_GBS, step_ms=v["step_ms"],
+                         step_gbs=v["step_bytes"] / (v["step_ms"] * 1e-3) / 1e9,
+                         stress_rel_err_vs_oracle=(rel(r["stress"][0], fix["stress_ave"])
+                                                   if fix is not None else None))
+    return out
+
+
+def workload_c3(args, mm, configs):
+    """One 100^3 RVE, one random strain."""
+    rows = 3 * 101 ** 3
+    out = {"workload": "c3: one 100^3 hex8 RVE (%d dofs), 4 fibers vf 0.4, strain ~ N(0, 1e-3)" % rows}
+    res = c3_solves(mm, configs, ("mixed_dict", "mixed_full", "fp64_dict"))
+    out.update(c3_report(res, c3_fixture()))
+    out["ell_api_spmv"] = ell_api_spmv(mm, configs.c3_spec())
     log("c3 reference-layout FP64 SpMV (ell_device_mvp): %.3f ms -> %.0f GB/s"
         % (out["ell_api_spmv"]["ms"], out["ell_api_spmv"]["gbs"]))
-    a, b = res["mixed_dict"]["stress"][0], res["fp64_dict"]["stress"][0]
-    out["mixed_vs_fp64_stress_rel"] = float(np.max(np.abs(a - b)) / np.max(np.abs(b)))
-    out["full_equals_dict"] = bool(np.array_equal(res["mixed_full"]["stress"][0], a))
+    a, b = res["mixed_dict"]["result"]["stress"][0], res["fp64_dict"]["result"]["stress"][0]
+    out["mixed_vs_fp64_stress_rel"] = rel(a, b)
+    out["full_equals_dict"] = bool(np.array_equal(res["mixed_full"]["result"]["stress"][0], a))
+    out["ceiling"] = stream_ceilings(mm)
     return out
 
 
@@ -152,48 +211,118 @@ def workload_c4(args, mm, configs):
         res = run_batch(eng, specs, mm)
         dt = time.perf_counter() - t
     its = [r["pcg_its"] for r in res]
+    errs = []
+    path = os.path.join(GOLD, "wl_c4.npz")
+    if os.path.exists(path):
+        g = np.load(path)
+        errs = [rel(res[i]["c_tangent"], c) for i, c in zip(g["index"], g["c_tangent"])]
     out = {"workload": "c4: 512 RVEs, n in {8,12,16,20,24,32}, 6 load cases each", "seconds": dt,
            "tangents_per_s": len(specs) / dt, "dofs": rows, "pcg_its": [min(its), max(its)],
-           "all_converged": all(r["status"] == 0 for r in res)}
+           "all_converged": all(r["status"] == 0 for r in res),
+           "max_tangent_rel_err_vs_oracle": max(errs) if errs else None,
+           "oracle_checked": len(errs)}
     log("c4: %.3f s -> %.1f tangents/s" % (dt, len(specs) / dt))
     return out
 
 
-def workload_c5(args, mm, configs):
-    """FE^2 serving stream, two Newton passes over the same gauss points."""
-    count = args.count or 4096
-    rate = args.rate or 600.0
-    stream = configs.c5_stream(count=count, rate=rate)
+def serve_stream(mm, se, stream, rid0, warm, strain_scale=1.0, limit_s=600):
+    """Submits the requests at their arrival times to a started engine and
+    collects the results; returns (results by request index, seconds)."""
     eye = [np.array([1e-3 if a == b else 0.0 for b in range(6)]) for a in range(6)]
-    passes = []
-    with mm.Engine(nrhs=7, precision=mm.MM_PREC_MIXED, hbm_budget=64 << 30, max_rves=count + 64,
-                   slab_nodes=256, inner_rtol=1e-5, rtol=1e-9, steps_per_launch=8) as se:
+    got = {}
+    t0 = time.perf_counter()
+    for i, rq in enumerate(stream):
+        while time.perf_counter() - t0 < rq["arrival"]:
+            for r in se.poll():
+                got[r["id"] - rid0] = r
+            time.sleep(0.0002)
+        rc = se.submit(rid0 + i, rq["spec"], eye + [strain_scale * rq["strain"]], mm.MM_TANGENT_UNIT,
+                       warm_key=rq.get("warm_key", 0), warm_start=1 if warm else 0)
+        if rc:
+            raise RuntimeError("submit failed: %d" % rc)
+    while len(got) < len(stream) and time.perf_counter() - t0 < limit_s:
+        for r in se.poll():
+            got[r["id"] - rid0] = r
+        time.sleep(0.0005)
+    return got, time.perf_counter() - t0
+
+
+def stream_stats(got, seconds):
+    rs = list(got.values())
+    lat = np.array([(r["t_done"] - r["t_submit"]) * 1e3 for r in rs])
+    its = np.array([r["pcg_its"] for r in rs])
+    return dict(requests=len(rs), seconds=seconds, tangents_per_s=len(rs) / seconds,
+                p50_ms=float(np.percentile(lat, 50)), p99_ms=float(np.percentile(lat, 99)),
+                mean_pcg_its=float(its.mean()), all_ok=all(r["status"] == 0 for r in rs))
+
+
+def c5_engine(mm, count):
+    return mm.Engine(nrhs=7, precision=mm.MM_PREC_MIXED, hbm_budget=48 << 30, max_rves=min(count + 64, 4096),
+                     slab_nodes=256, inner_rtol=1e-5, rtol=1e-9, steps_per_launch=8)
+
+
+def workload_c5(args, mm, configs):
+    """FE^2 serving stream: a sweep of arrival rates for the saturation knee,
+    then the full stream with a warm-started second Newton pass."""
+    sweep = []
+    count = args.count or 4096
+    rates = [float(r) for r in args.rates.split(",")] if args.rates else [1000.0, 2000.0, 3000.0, 4000.0, 6000.0]
+    for rate in rates:
+        stream = configs.c5_stream(count=count, rate=rate)
+        with c5_engine(mm, count) as se:
+            se.start()
+            got, ts = serve_stream(mm, se, stream, 1, False)
+            se.stop()
+        st = stream_stats(got, ts)
+        st["rate"] = rate
+        sweep.append(st)
+        log("c5 rate %.0f: %s" % (rate, st))
+    knee = max((s for s in sweep if s["tangents_per_s"] >= 0.95 * s["rate"] and s["all_ok"]),
+               key=lambda s: s["rate"], default=None)
+    capacity = max(s["tangents_per_s"] for s in sweep)
+    out = {"workload": "c5: FE^2 stream (12^3/20^3/30^3 RVEs), strain + 6 unit loads per request, Poisson "
+                       "arrivals", "rate_sweep": sweep, "capacity_tangents_per_s": capacity,
+           "knee": knee}
+    # the full stream, a 64-request sample kept resident and warm-started
+    fix_path = os.path.join(GOLD, "wl_c5.npz")
+    g = np.load(fix_path) if os.path.exists(fix_path) else None
+    full_rate = args.rate or (knee["rate"] if knee else 2000.0)
+    stream = configs.c5_stream(rate=full_rate)
+    sample = [int(i) for i in g["index"]] if g is not None else list(range(37, len(stream), 512))
+    for i in sample:
+        stream[i]["warm_key"] = stream[i]["key"]
+    with c5_engine(mm, len(stream)) as se:
         se.start()
-        for ps in range(2):
-            got = []
-            t0 = time.perf_counter()
-            for i, rq in enumerate(stream):
-                while time.perf_counter() - t0 < rq["arrival"]:
-                    got.extend(se.poll())
-                    time.sleep(0.0002)
-                strain = rq["strain"] * (1.0 + 0.05 * ps)  # next Newton iterate
-                se.submit(ps * count + i + 1, rq["spec"], eye + [strain], mm.MM_TANGENT_UNIT,
-                          warm_key=rq["key"], warm_start=ps)
-            while len(got) < count and time.perf_counter() - t0 < 600:
-                got.extend(se.poll())
-                time.sleep(0.0005)
-            ts = time.perf_counter() - t0
-            lat = np.array([(r["t_done"] - r["t_submit"]) * 1e3 for r in got])
-            its = np.array([r["pcg_its"] for r in got])
-            passes.append(dict(requests=len(got), seconds=ts, tangents_per_s=len(got) / ts,
-                               p50_ms=float(np.percentile(lat, 50)), p99_ms=float(np.percentile(lat, 99)),
-                               mean_pcg_its=float(its.mean()),
-                               all_ok=all(r["status"] == 0 for r in got)))
-            log("c5 pass %d: %s" % (ps, passes[-1]))
+        got, ts = serve_stream(mm, se, stream, 1, False)
+        cold = stream_stats(got, ts)
+        log("c5 full stream at %.0f/s: %s" % (full_rate, cold))
+        sub = [dict(stream[i], arrival=0.0) for i in sample]
+        wgot, wts = serve_stream(mm, se, sub, 1000000, True, strain_scale=1.05)
+        warm = stream_stats(wgot, wts)
         se.stop()
-    return {"workload": "c5: FE^2 stream, %d gauss points (12^3/20^3/30^3 RVEs), Poisson %.0f req/s, "
-                        "strain + 6 unit loads per request; pass 2 warm-started" % (count, rate),
-            "cold": passes[0], "warm": passes[1]}
+    cold_its = float(np.mean([got[i]["pcg_its"] for i in sample]))
+    warm["mean_pcg_its_cold_same_requests"] = cold_its
+    log("c5 warm pass over the sample: %s" % warm)
+    errs = []
+    if g is not None:
+        for j, i in enumerate(sample):
+            errs.append(rel(got[i]["stress"][6], g["stress_ave"][j]))
+            errs.append(rel(got[i]["c_tangent"], g["c_tangent"][j]))
+            errs.append(rel(wgot[j]["stress"][6], 1.05 * g["stress_ave"][j]))
+    out.update(full_stream=dict(cold, rate=full_rate), warm_pass=warm,
+               max_rel_err_vs_oracle=max(errs) if errs else None, oracle_checked=len(sample) if errs else 0)
+    return out
+
+
+def cpu_model():
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for ln in out.splitlines():
+            if ln.startswith("Model name:"):
+                return ln.split(":", 1)[1].strip()
+    except (OSError, subprocess.SubprocessError):
+        pass
+    return "unknown CPU"
 
 
 def main():
@@ -208,15 +337,19 @@ def main():
     ap.add_argument("--rtol", type=float, default=1e-9, help="FP64 residual target (relative)")
     ap.add_argument("--tune-batch", type=int, default=0, help="RVEs the autotuner solves (0 = the batch)")
     ap.add_argument("--spl", type=int, default=16, help="PCG steps per graph launch")
-    ap.add_argument("--cpu-sample", type=int, default=2, help="RVE tangents timed on the CPU")
+    ap.add_argument("--cpu-sample", type=int, default=2, help="RVE tangents timed on one CPU core")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-c3", action="store_true", help="skip the 100^3 leg")
+    ap.add_argument("--no-fp64", action="store_true", help="skip the batched FP64 leg")
     ap.add_argument("--serve", type=int, default=1024, help="requests in the serving sample")
+    ap.add_argument("--serve-rate", type=float, default=2000.0, help="arrival rate of the serving sample")
     ap.add_argument("--no-full-ell", action="store_true", help="skip the full-ELL SpMV figure")
     ap.add_argument("--kernels-only", action="store_true")
     ap.add_argument("--workload", default="c2", choices=("c2", "c3", "c4", "c5"))
-    ap.add_argument("--rate", type=float, default=0.0, help="c5 arrival rate (requests/s)")
-    ap.add_argument("--count", type=int, default=0, help="c5 requests per pass")
+    ap.add_argument("--rate", type=float, default=0.0, help="c5 full-stream arrival rate (requests/s)")
+    ap.add_argument("--rates", default="", help="c5 sweep rates, comma separated")
+    ap.add_argument("--count", type=int, default=0, help="c5 requests per sweep point")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: measures one GPU; multi-GPU runs are not part of it "
@@ -233,7 +366,6 @@ def main():
 
     B, n = args.batch, args.n
     specs = [configs.c2_spec(2, i, n=n) for i in range(B)]
-    tuned = None
     if args.slab == 0 or args.inner_rtol == 0.0:
         from madmax_amd import autotune
         t = time.perf_counter()
@@ -248,20 +380,23 @@ def main():
     per_rve = rows * 900 + (1 << 20)
     budget = int(per_rve * B * 1.3) + (256 << 20)
     log("batch %d x %d^3 (%d rows each), arena %.1f GB" % (B, n, rows, budget / 1e9))
-    eng = mm.Engine(nrhs=6, precision=mm.MM_PREC_MIXED, hbm_budget=budget, max_rves=B + 8,
-                    slab_nodes=args.slab, inner_rtol=args.inner_rtol, rtol=args.rtol,
-                    steps_per_launch=args.spl, use_graph=True, ell_storage=mm.MM_ELL_DICT)
+
+    def engine(**kw):
+        cfg = dict(nrhs=6, precision=mm.MM_PREC_MIXED, hbm_budget=budget, max_rves=B + 8, slab_nodes=args.slab,
+                   inner_rtol=args.inner_rtol, rtol=args.rtol, steps_per_launch=args.spl, use_graph=True,
+                   ell_storage=mm.MM_ELL_DICT)
+        cfg.update(kw)
+        return mm.Engine(**cfg)
+
+    eng = engine()
     if args.kernels_only:
-        import ctypes
         run_batch(eng, specs, mm, warm=True)
         step_ms, step_bytes = eng.time_step(reps=20)
         spmv_ms, spmv_bytes = eng.time_spmv(reps=20)
         eng.close()
-        copy_ms = ctypes.c_double()
-        mm.lib().mm_hbm_copy_probe(0, 1 << 30, 5, ctypes.byref(copy_ms))
+        ceil = stream_ceilings(mm, nbytes=1 << 30, reps=5)
         print(json.dumps({"step_ms": step_ms, "step_bytes": step_bytes, "spmv_ms": spmv_ms,
-                          "spmv_bytes": spmv_bytes, "copy_bytes_each_way": 1 << 30,
-                          "copy_ms": copy_ms.value}), flush=True)
+                          "spmv_bytes": spmv_bytes, "ceiling": ceil}), flush=True)
         return
     for w in range(args.warmup):
         t = time.perf_counter()
@@ -280,23 +415,30 @@ def main():
     log("timed: %.3f s for %d steps -> %.1f tangents/s (its %d..%d, sweeps %d..%d)"
         % (elapsed, args.steps, value, min(its), max(its), min(sweeps), max(sweeps)))
 
-    # roofline of the dominant kernel (the PCG step's SpMV + update) and the
-    # SpMV alone, on the resident batch
+    # stand-alone replicas of the step kernel and of the SpMV on the resident batch
     run_batch(eng, specs, mm, warm=True)
-    step_ms, step_bytes = eng.time_step(reps=50)
-    achieved = step_bytes / (step_ms * 1e-3) / 1e9
-    log("step kernel: %.3f ms, %.3f GB/launch -> %.0f GB/s" % (step_ms, step_bytes / 1e9, achieved))
+    rep_ms, step_bytes = eng.time_step(reps=50)
     spmv_ms, spmv_bytes = eng.time_spmv(reps=50)
     spmv_gbs = spmv_bytes / (spmv_ms * 1e-3) / 1e9
-    log("spmv (dictionary ELL): %.3f ms, %.3f GB/launch -> %.0f GB/s" % (spmv_ms, spmv_bytes / 1e9,
-                                                                         spmv_gbs))
+    log("step kernel replica: %.3f ms, %.3f GB/launch; spmv (dictionary ELL): %.3f ms -> %.0f GB/s"
+        % (rep_ms, step_bytes / 1e9, spmv_ms, spmv_gbs))
     eng.close()
+
+    # kernel A inside a solve of the batch: every launch timed (one stream, no graph)
+    pe = engine(profile_step_kernel=True)
+    pres = run_batch(pe, specs, mm)
+    a_ms = pe.step_kernel_times()
+    pe.close()
+    in_solve_ms = float(np.median(a_ms))
+    same_prof = all(np.array_equal(a["c_tangent"], b["c_tangent"]) for a, b in zip(pres, results))
+    achieved = step_bytes / (in_solve_ms * 1e-3) / 1e9
+    log("kernel A in the solve: median %.3f ms over %d launches -> %.0f GB/s (tangents equal: %s)"
+        % (in_solve_ms, len(a_ms), achieved, same_prof))
+
     full = None
     if not args.no_full_ell:
         fb = int((rows * (81 * 4 + 27 * 4 / 3.0) + per_rve) * B * 1.3) + (256 << 20)
-        fe = mm.Engine(nrhs=6, precision=mm.MM_PREC_MIXED, hbm_budget=fb, max_rves=B + 8,
-                       slab_nodes=args.slab, inner_rtol=args.inner_rtol, rtol=args.rtol,
-                       steps_per_launch=args.spl, use_graph=True, ell_storage=mm.MM_ELL_FULL)
+        fe = engine(hbm_budget=fb, ell_storage=mm.MM_ELL_FULL)
         t = time.perf_counter()
         fres = run_batch(fe, specs, mm, warm=True)
         tf = time.perf_counter() - t
@@ -305,126 +447,132 @@ def main():
         same = all(np.array_equal(a["c_tangent"], b["c_tangent"]) for a, b in zip(fres, results))
         full = dict(ms=f_ms, bytes=f_bytes, gbs=f_bytes / (f_ms * 1e-3) / 1e9,
                     frac=f_bytes / (f_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
-                    solve_tangents_per_s=B / tf, tangents_equal_to_dictionary_run=same)
-        log("spmv (full ELL): %.3f ms, %.3f GB/launch -> %.0f GB/s; batch in %.3f s, same tangents: %s"
-            % (f_ms, f_bytes / 1e9, full["gbs"], tf, same))
-    import ctypes
-    copy_ms = ctypes.c_double()
-    copy_bytes = 4 << 30
-    mm.lib().mm_hbm_copy_probe(0, copy_bytes, 10, ctypes.byref(copy_ms))
-    copy_gbs = 2 * copy_bytes / (copy_ms.value * 1e-3) / 1e9
-    log("device copy: %.0f GB/s (read + write of 4 GiB)" % copy_gbs)
+                    solve_tangents_per_s=B / tf, tangents_equal_to_dictionary_run=same,
+                    note="%d x 30^3 RVEs, 6 loads; 4.7 GB per launch, above the 256 MB last-level cache" % B)
+        log("spmv (full ELL): %.3f ms -> %.0f GB/s; batch in %.3f s, same tangents: %s"
+            % (f_ms, full["gbs"], tf, same))
+
+    # the same batch in FP64 (FP64 PCG to the same outer tolerance)
+    fp64 = None
+    if not args.no_fp64:
+        de = engine(precision=mm.MM_PREC_FP64, inner_rtol=0.1 * args.rtol, max_sweeps=2)
+        run_batch(de, specs, mm)
+        t = time.perf_counter()
+        nrep = 2
+        for _ in range(nrep):
+            dres = run_batch(de, specs, mm)
+        td = time.perf_counter() - t
+        de.close()
+        fp64 = dict(value=B * nrep / td, unit="RVE tangents/s", pcg_its=[min(r["pcg_its"] for r in dres),
+                                                                        max(r["pcg_its"] for r in dres)],
+                    all_converged=all(r["status"] == 0 for r in dres),
+                    max_rel_diff_vs_mixed=max(rel(a["c_tangent"], b["c_tangent"]) for a, b in zip(dres, results)))
+        log("batched fp64: %.1f tangents/s" % fp64["value"])
+
+    # the 100^3 RVE: HBM-resident SpMV of both layouts and the mixed solve
+    c3 = None
+    if not args.no_c3:
+        res3 = c3_solves(mm, configs, ("mixed_full", "mixed_dict"))
+        c3 = c3_report(res3, c3_fixture())
+        c3["full_equals_dict"] = bool(np.array_equal(res3["mixed_full"]["result"]["stress"][0],
+                                                     res3["mixed_dict"]["result"]["stress"][0]))
+        c3["ell_api_fp64"] = ell_api_spmv(mm, configs.c3_spec())
+        log("c3 reference-layout FP64 SpMV (ell_device_mvp): %.3f ms -> %.0f GB/s"
+            % (c3["ell_api_fp64"]["ms"], c3["ell_api_fp64"]["gbs"]))
+
+    ceiling = stream_ceilings(mm)
+    log("streaming kernels: read %.0f GB/s, copy %.0f GB/s" % (ceiling["read_gbs"] or 0, ceiling["copy_gbs"] or 0))
 
     # plain FP64 single-RVE GPU path: the reference's algorithm (zero interior
-    # start, one load case at a time), host check per iteration
+    # start, one load case at a time, Newton limits of micro/src/init.c:7-8)
     plain = None
     if not args.no_plain:
-        pe = mm.Engine(nrhs=1, precision=mm.MM_PREC_FP64, hbm_budget=per_rve * 2 + (64 << 20),
-                       max_rves=2, slab_nodes=128, inner_rtol=1e-10, rtol=0.0, abs_tol=1e-4,
-                       max_sweeps=2, sync_every_step=True, use_graph=False,
-                       initial_guess=mm.MM_GUESS_ZERO)
+        plain = {}
         nplain = 2
-        t = time.perf_counter()
-        cs = []
-        for j in range(nplain):
-            sig = []
-            for i, load in enumerate(mm.unit_loads()):
-                pe.submit(10 * j + i + 1, specs[j], [load], mm.MM_TANGENT_NONE, warm_key=j + 1)
+        for name, kw in (("host_sync_per_step", dict(sync_every_step=True, use_graph=False)),
+                         ("graph_replay", dict(sync_every_step=False, use_graph=True, steps_per_launch=16))):
+            pe = mm.Engine(nrhs=1, precision=mm.MM_PREC_FP64, hbm_budget=per_rve * 2 + (64 << 20),
+                           max_rves=2, slab_nodes=128, inner_rtol=1e-10, rtol=0.0, abs_tol=1e-4,
+                           max_sweeps=2, initial_guess=mm.MM_GUESS_ZERO, **kw)
+            for load in mm.unit_loads()[:1]:  # warm-up (graph capture)
+                pe.submit(99, specs[0], [load], mm.MM_TANGENT_NONE, warm_key=99)
                 pe.drain()
-                sig.append(pe.poll()[0]["stress"][0] / 1e-3)
-            cs.append(np.array(sig).T)
-        tp = time.perf_counter() - t
-        pe.close()
-        plain = dict(value=nplain / tp, unit="RVE tangents/s", n=nplain)
-        log("plain fp64: %.2f tangents/s" % plain["value"])
+                pe.poll()
+            t = time.perf_counter()
+            cs = []
+            for j in range(nplain):
+                sig = []
+                for i, load in enumerate(mm.unit_loads()):
+                    pe.submit(10 * j + i + 1, specs[j], [load], mm.MM_TANGENT_NONE, warm_key=j + 1)
+                    pe.drain()
+                    sig.append(pe.poll()[0]["stress"][0] / 1e-3)
+                cs.append(np.array(sig).T)
+            tp = time.perf_counter() - t
+            pe.close()
+            plain[name] = dict(value=nplain / tp, unit="RVE tangents/s", n=nplain,
+                               err_vs_batched=[rel(c, results[j]["c_tangent"]) for j, c in enumerate(cs)])
+            log("plain fp64 (%s): %.2f tangents/s" % (name, plain[name]["value"]))
 
-    # CPU baseline: the FP64 oracle, one core, one RVE at a time (bounded sample)
-    cpu = None
-    cpu_par = None
-    acc = None
+    # CPU baselines: the FP64 oracle, one core (bounded sample), then every
+    # usable core with one RVE per thread; accuracy over all of them
+    cpu = cpu_par = None
+    errs = []
     if not args.no_cpu:
         from oracle import pyoracle as po
-        model = "unknown CPU"
-        try:
-            with open("/proc/cpuinfo") as f:
-                model = next((ln.split(":", 1)[1].strip() for ln in f if ln.startswith("model name")), model)
-        except OSError:
-            pass
-        errs = []
-        t = time.perf_counter()
-        for j in range(args.cpu_sample):
-            s = specs[j]
+        model = cpu_model()
+
+        def oracle_tangent(j):
+            s = specs[j % len(specs)]
             m = po.mesh(3, n, n, n)
             ro = po.Rve(3, n, po.phase_map(m, mm.spec_fibers(s)), [s.young[0], s.young[1]],
                         [s.poisson[0], s.poisson[1]])
             c_ref, _ = ro.c_tangent_unit(1e-3, 1e-10)
-            errs.append(float(np.max(np.abs(results[j]["c_tangent"] - c_ref)) / np.max(np.abs(c_ref))))
-            if plain is not None and j < len(cs):
-                plain.setdefault("err", []).append(
-                    float(np.max(np.abs(cs[j] - c_ref)) / np.max(np.abs(c_ref))))
+            return c_ref
+
+        t = time.perf_counter()
+        for j in range(args.cpu_sample):
+            errs.append(rel(results[j]["c_tangent"], oracle_tangent(j)))
         tc = time.perf_counter() - t
         cpu = dict(value=args.cpu_sample / tc, unit="RVE tangents/s", cores=1, kind="port",
                    sample="%d seeded %d^3 RVE tangents (assembly + 6 FP64 Jacobi-PCG solves to 1e-10 "
                           "+ averaging), oracle/oracle.c gcc -O3, one thread" % (args.cpu_sample, n),
                    cpu_model=model)
-        acc = max(errs)
-        log("cpu oracle: %.3f tangents/s, max tangent err %.2e" % (cpu["value"], acc))
+        log("cpu oracle: %.3f tangents/s" % cpu["value"])
 
-        # the same oracle on P cores, one RVE per thread (the C calls release
-        # the GIL); P = the usable cores, at most 16
         from concurrent.futures import ThreadPoolExecutor
-        P = max(1, min(16, len(os.sched_getaffinity(0))))
-
-        def one(j):
-            s = specs[j % len(specs)]
-            m = po.mesh(3, n, n, n)
-            ro = po.Rve(3, n, po.phase_map(m, mm.spec_fibers(s)), [s.young[0], s.young[1]],
-                        [s.poisson[0], s.poisson[1]])
-            ro.c_tangent_unit(1e-3, 1e-10)
-
+        P = len(os.sched_getaffinity(0))
+        idx = list(range(args.cpu_sample, args.cpu_sample + P))
         t = time.perf_counter()
         with ThreadPoolExecutor(P) as ex:
-            list(ex.map(one, range(P)))
+            refs = list(ex.map(oracle_tangent, idx))
         tp_ = time.perf_counter() - t
+        for j, c_ref in zip(idx, refs):
+            errs.append(rel(results[j % B]["c_tangent"], c_ref))
         cpu_par = dict(value=P / tp_, unit="RVE tangents/s", cores=P, kind="port",
-                       sample="%d seeded %d^3 RVE tangents, one per thread, same oracle" % (P, n),
-                       cpu_model=model)
-        log("cpu oracle, %d threads: %.3f tangents/s" % (P, cpu_par["value"]))
+                       sample="%d seeded %d^3 RVE tangents, one per thread on every usable core, same oracle"
+                              % (P, n), cpu_model=model, os_cpu_count=os.cpu_count())
+        log("cpu oracle, %d threads: %.3f tangents/s; max tangent err over %d RVEs %.2e"
+            % (P, cpu_par["value"], len(errs), max(errs)))
 
     # the reference ELL API's SpMV (FP64, row-major slot order) on one RVE of
     # the batch; its 87 MB matrix fits in the last-level cache
     ell_api = ell_api_spmv(mm, specs[0])
     ell_api["note"] = "one 30^3 RVE, FP64 + int32, 81 slots/row; matrix (87 MB) cache-resident"
-    log("reference-layout FP64 SpMV (ell_device_mvp): %.3f ms -> %.0f GB/s" % (ell_api["ms"], ell_api["gbs"]))
+    log("reference-layout FP64 SpMV (ell_device_mvp), 30^3: %.3f ms -> %.0f GB/s" % (ell_api["ms"], ell_api["gbs"]))
 
     # serving sample (FE^2 stream, mixed RVE sizes, Poisson arrivals)
     serving = None
     if args.serve > 0:
-        stream = configs.c5_stream(count=args.serve, rate=600.0)
-        se = mm.Engine(nrhs=7, precision=mm.MM_PREC_MIXED, hbm_budget=8 << 30, max_rves=512,
-                       slab_nodes=args.slab, inner_rtol=args.inner_rtol, rtol=args.rtol,
-                       steps_per_launch=8)
-        se.start()
-        t0s = time.perf_counter()
-        got = []
-        for i, rq in enumerate(stream):
-            while time.perf_counter() - t0s < rq["arrival"]:
-                got.extend(se.poll())
-                time.sleep(0.0002)
-            loads = [np.array([1e-3 if a == b else 0.0 for b in range(6)]) for a in range(6)]
-            se.submit(i + 1, rq["spec"], loads + [rq["strain"]], mm.MM_TANGENT_UNIT)
-        while len(got) < len(stream) and time.perf_counter() - t0s < 300:
-            got.extend(se.poll())
-            time.sleep(0.0005)
-        ts = time.perf_counter() - t0s
-        se.stop()
-        se.close()
-        lat = np.array([(r["t_done"] - r["t_submit"]) * 1e3 for r in got])
-        serving = dict(requests=len(got), tangents_per_s=len(got) / ts,
-                       p50_ms=float(np.percentile(lat, 50)), p99_ms=float(np.percentile(lat, 99)),
-                       all_ok=all(r["status"] == 0 for r in got))
+        stream = configs.c5_stream(count=args.serve, rate=args.serve_rate)
+        with c5_engine(mm, args.serve) as se:
+            se.start()
+            got, ts = serve_stream(mm, se, stream, 1, False, limit_s=300)
+            se.stop()
+        serving = stream_stats(got, ts)
+        serving["rate"] = args.serve_rate
         log("serving: %s" % serving)
 
+    traffic = measured_traffic(B, n)
     line = {
         "metric": METRIC,
         "value": value,
@@ -443,20 +591,31 @@ def main():
                    "batch": B, "slab_nodes": args.slab, "inner_rtol": args.inner_rtol,
                    "rtol": args.rtol, "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(B, n),
-                     "kernel": "k_inner_time<float,6,true>: the PCG step kernel's SpMV + p/q update "
-                               "(dictionary ELL), %d resident RVEs" % B,
-                     "avg_ms": step_ms, "bytes_per_launch": step_bytes,
-                     "copy_gbs_measured": copy_gbs},
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "traffic_source": ("constant from profiles/pmc_k_inner_time_128x30cube.md (rocprofv3 "
+                                        "FETCH_SIZE/WRITE_SIZE of the replica, copy-calibrated)"
+                                        if traffic else None),
+                     "kernel": "k_step_a<float,6,true> (kernel A: SpMV + p/q update, dictionary ELL) inside a "
+                               "solve of %d RVEs: median of %d launches" % (B, len(a_ms)),
+                     "avg_ms": in_solve_ms, "bytes_per_launch": step_bytes,
+                     "replica_ms": rep_ms, "replica_frac": step_bytes / (rep_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                     "ceiling_gbs": ceiling},
         "spmv": {"dictionary": {"ms": spmv_ms, "bytes": spmv_bytes, "gbs": spmv_gbs,
                                 "frac": spmv_gbs / HBM_PEAK_GBS},
-                 "full_ell": full, "ell_api_fp64": ell_api},
+                 "full_ell": full, "ell_api_fp64_30cube": ell_api,
+                 "c3_full_ell_fp32": ({k: c3["mixed_full"][k] for k in ("spmv_ms", "spmv_bytes", "spmv_gbs",
+                                                                         "spmv_frac")} if c3 else None),
+                 "c3_ell_api_fp64": c3["ell_api_fp64"] if c3 else None},
+        "c3": c3,
         "cpu_baseline": cpu,
         "cpu_baseline_threads": cpu_par,
+        "fp64_batched": fp64,
         "plain_fp64_gpu": plain,
-        "speedup_vs_plain": (value / plain["value"]) if plain else None,
-        "accuracy": {"max_rel_err_tangent": acc, "all_converged": status_ok,
-                     "pcg_its": [min(its), max(its)], "ir_sweeps": [min(sweeps), max(sweeps)]},
+        "speedup_vs_plain": ({k: value / v["value"] for k, v in plain.items()} if plain else None),
+        "speedup_vs_fp64_batched": (value / fp64["value"]) if fp64 else None,
+        "accuracy": {"max_rel_err_tangent": max(errs) if errs else None, "rves_checked": len(errs),
+                     "all_converged": status_ok, "pcg_its": [min(its), max(its)],
+                     "ir_sweeps": [min(sweeps), max(sweeps)]},
         "serving": serving,
     }
     print(json.dumps(line), flush=True)

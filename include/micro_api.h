@@ -173,20 +173,29 @@ typedef struct {
   int initial_guess;     /* MM_GUESS_ZERO or MM_GUESS_AFFINE (default AFFINE) */
   int solver_streams;    /* 1, or 2 (default): resident RVEs stepped in two
                             groups on two HIP streams */
+  int profile_step_kernel; /* 1: no graph, one stream, and every launch of the
+                            PCG step kernel (kernel A: SpMV + p/q update)
+                            timed with HIP events (mm_engine_step_kernel_times) */
 } mm_engine_config;
 
 typedef struct {
   uint64_t id;
   mm_rve_spec rve;
-  int nload;                      /* == engine nrhs */
+  int nload;                      /* 1 .. engine nrhs */
   double strain[MM_MAX_RHS][MM_NVOI];
   int tangent_mode;               /* MM_TANGENT_* */
   uint64_t warm_key;              /* 0: nothing kept; else the RVE (mesh, ELL,
                                      displacements) stays resident under this
                                      key and a later request with the same key
                                      and geometry reuses it without assembly */
-  int warm_start;                 /* with a reused RVE: 1 starts from the kept
-                                     displacements, 0 from zero */
+  int warm_start;                 /* with a reused RVE: 0 starts cold (as the
+                                     engine's initial_guess); 1 starts from the
+                                     kept displacements: if six kept load cases
+                                     span the strain space, load k starts from
+                                     their combination that matches its strain
+                                     (exact for linear materials), else from
+                                     kept load k plus the affine field of the
+                                     strain increment */
 } mm_request;
 
 typedef struct {
@@ -204,6 +213,14 @@ typedef struct {
 typedef struct mm_engine mm_engine;
 
 int mm_engine_config_default(mm_engine_config *cfg);
+/* Host-side check of a request for an engine with nrhs load cases (no GPU
+ * call; mm_engine_submit runs it first).  Returns MM_ERR_ARG for an invalid
+ * RVE, non-finite strains, a load count outside 1..nrhs, and for load cases
+ * that do not have the shape the tangent mode divides by: MM_TANGENT_UNIT
+ * needs load i = a_i e_i (a_i != 0, every other component exactly 0) for
+ * i = 0..5; MM_TANGENT_FD needs load i+1 to differ from load 0 in component
+ * i and in no other (micro/src/homogenize.c:59-94). */
+int mm_request_validate(const mm_request *req, int nrhs);
 int mm_engine_create(const mm_engine_config *cfg, mm_engine **eng);
 int mm_engine_destroy(mm_engine *eng);
 /* Queues a request (thread-safe); returns MM_REJECTED if it can never fit:
@@ -234,6 +251,10 @@ typedef struct {
   double spmv_ms;     /* average duration of the ELL SpMV kernel, from HIP events */
 } mm_engine_info;
 int mm_engine_info_get(mm_engine *eng, mm_engine_info *info);
+/* Durations (ms) of the step-kernel launches recorded since the last call
+ * by an engine created with profile_step_kernel (oldest first, at most max);
+ * *n receives the count and the record is cleared. */
+int mm_engine_step_kernel_times(mm_engine *eng, double *ms, int max, int *n);
 
 /* Inspection of an RVE kept resident under warm_key (finished requests):
  * the Jacobian in the reference's row-major ELL layout (nrow x 81, values
@@ -258,6 +279,13 @@ int mm_engine_time_step(mm_engine *eng, int reps, double *avg_ms, double *bytes)
  * practical HBM ceiling, and the known-size reference that calibrates the
  * FETCH_SIZE / WRITE_SIZE counters. */
 int mm_hbm_copy_probe(int device, size_t bytes, int reps, double *avg_ms);
+
+/* Hand-written streaming kernels over `bytes` bytes (16-byte non-temporal
+ * accesses, grid-stride, 8 workgroups per CU), timed with HIP events over
+ * `reps` launches after one warm-up: mode 0 copies (reads and writes `bytes`
+ * each), mode 1 only reads.  The HBM rate a kernel of that access pattern
+ * can reach on this device. */
+int mm_hbm_stream_probe(int device, size_t bytes, int reps, int mode, double *avg_ms);
 
 #ifdef __cplusplus
 }

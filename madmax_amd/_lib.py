@@ -77,7 +77,8 @@ class EngineConfig(C.Structure):
                 ("max_inner_its", C.c_int), ("max_sweeps", C.c_int),
                 ("steps_per_launch", C.c_int), ("use_graph", C.c_int),
                 ("sync_every_step", C.c_int), ("ell_storage", C.c_int),
-                ("initial_guess", C.c_int), ("solver_streams", C.c_int)]
+                ("initial_guess", C.c_int), ("solver_streams", C.c_int),
+                ("profile_step_kernel", C.c_int)]
 
 
 class Request(C.Structure):
@@ -142,6 +143,7 @@ SIGNATURES = {
     "micro_get_elem_type": (C.c_int, [ip]),
     "micro_get_displacement": (C.c_int, [dp]),
     "mm_engine_config_default": (C.c_int, [C.POINTER(EngineConfig)]),
+    "mm_request_validate": (C.c_int, [C.POINTER(Request), C.c_int]),
     "mm_engine_create": (C.c_int, [C.POINTER(EngineConfig), C.POINTER(C.c_void_p)]),
     "mm_engine_destroy": (C.c_int, [C.c_void_p]),
     "mm_engine_submit": (C.c_int, [C.c_void_p, C.POINTER(Request)]),
@@ -152,6 +154,7 @@ SIGNATURES = {
     "mm_engine_stop": (C.c_int, [C.c_void_p]),
     "mm_engine_now": (C.c_double, [C.c_void_p]),
     "mm_engine_info_get": (C.c_int, [C.c_void_p, C.POINTER(EngineInfo)]),
+    "mm_engine_step_kernel_times": (C.c_int, [C.c_void_p, dp, C.c_int, ip]),
     "mm_engine_rve_sizes": (C.c_int, [C.c_void_p, C.c_uint64, ip, ip]),
     "mm_engine_download_ell": (C.c_int, [C.c_void_p, C.c_uint64, ip, dp]),
     "mm_engine_download_phase": (C.c_int, [C.c_void_p, C.c_uint64, ip]),
@@ -159,6 +162,7 @@ SIGNATURES = {
     "mm_engine_time_spmv": (C.c_int, [C.c_void_p, C.c_int, dp, dp]),
     "mm_engine_time_step": (C.c_int, [C.c_void_p, C.c_int, dp, dp]),
     "mm_hbm_copy_probe": (C.c_int, [C.c_int, C.c_size_t, C.c_int, dp]),
+    "mm_hbm_stream_probe": (C.c_int, [C.c_int, C.c_size_t, C.c_int, C.c_int, dp]),
 }
 
 _LIB = None

@@ -80,7 +80,9 @@ def tune(specs, nrhs=6, precision=_lib.MM_PREC_MIXED, rtol=1e-9, budget=None, us
         if r["ok"] and (best is None or r["step_ms"] < best["step_ms"]):
             best = r
     sn = best["slab_nodes"] if best else 128
-    if precision == _lib.MM_PREC_MIXED:
+    # the tolerance sweep compares against a converged slab-shape trial; with
+    # none, the defaults stand
+    if precision == _lib.MM_PREC_MIXED and best is not None:
         for tol in INNER_RTOLS:
             if tol == BASE_INNER:
                 continue

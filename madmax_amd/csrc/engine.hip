@@ -76,6 +76,9 @@ enum SlotState { SL_FREE = 0, SL_ACTIVE = 1, SL_CACHED = 2 };
 struct HostSlot {
   int state = SL_FREE;
   mm_request req;
+  // load cases of the request that last left displacements in the slot
+  double prev_load[MM_MAX_RHS][MM_NVOI];
+  int prev_nload = 0;
   long long first_slab = 0, nslab = 0, elem_base = 0, nelm = 0;
   int nrow = 0;
   double t_submit = 0, t_admit = 0;
@@ -219,6 +222,11 @@ struct mm_engine {
   std::atomic<bool> running{false};
   std::chrono::steady_clock::time_point t0;
   int last_error = 0;
+  // profile_step_kernel: event pairs around the kernel A launches queued in
+  // the current step, and the durations recorded so far
+  std::vector<hipEvent_t> prof_ev;
+  int prof_pending = 0;
+  std::vector<double> prof_ms;
 
   EngPtrs ptrs() const
   {
@@ -276,6 +284,8 @@ struct mm_engine {
 
 static void destroy_streams(mm_engine *e)
 {
+  for (hipEvent_t ev : e->prof_ev) (void)hipEventDestroy(ev);
+  e->prof_ev.clear();
   if (e->ev_fork) (void)hipEventDestroy(e->ev_fork);
   if (e->ev_join) (void)hipEventDestroy(e->ev_join);
   if (e->stream2) (void)hipStreamDestroy(e->stream2);
@@ -321,6 +331,7 @@ extern "C" int mm_engine_config_default(mm_engine_config *c)
   c->ell_storage = MM_ELL_DICT;
   c->initial_guess = MM_GUESS_AFFINE;
   c->solver_streams = 2;
+  c->profile_step_kernel = 0;
   return 0;
 }
 
@@ -352,6 +363,10 @@ extern "C" int mm_engine_create(const mm_engine_config *cfg, mm_engine **out)
   if (e->cfg.sync_every_step) {
     e->cfg.steps_per_launch = 1;
     e->cfg.use_graph = 0;
+  }
+  if (e->cfg.profile_step_kernel) {  // every kernel A launch bracketed by events
+    e->cfg.use_graph = 0;
+    e->cfg.solver_streams = 1;
   }
   e->SN = cfg->slab_nodes ? cfg->slab_nodes : 128;
   e->K = cfg->nrhs;
@@ -495,10 +510,20 @@ static void footprint(const mm_engine *e, const mm_rve_spec &s, int *nrow, long 
   *nelm = (long long)s.nex * s.ney * s.nez;
 }
 
-extern "C" int mm_engine_submit(mm_engine *e, const mm_request *req)
+// The tangent of a block is formed column by column from the load cases
+// (finish_rve): UNIT divides the stress of load i by its strain component i,
+// FD divides the difference of loads i+1 and 0 by their difference in
+// component i (micro/src/homogenize.c:59-94).  Both are right only for loads
+// of exactly that shape, so any other load is refused here instead of
+// returning a wrong tangent with status MM_OK.
+extern "C" int mm_request_validate(const mm_request *req, int nrhs)
 {
-  if (!e || !req) return MM_ERR_ARG;
-  if (!spec_valid(req->rve) || req->nload < 1 || req->nload > e->K) return MM_ERR_ARG;
+  if (!req) return MM_ERR_ARG;
+  if (nrhs < 1 || nrhs > MM_MAX_RHS) return MM_ERR_ARG;
+  if (!spec_valid(req->rve) || req->nload < 1 || req->nload > nrhs) return MM_ERR_ARG;
+  if (req->tangent_mode != MM_TANGENT_NONE && req->tangent_mode != MM_TANGENT_UNIT &&
+      req->tangent_mode != MM_TANGENT_FD)
+    return MM_ERR_ARG;
   if (req->tangent_mode == MM_TANGENT_UNIT && req->nload < 6) return MM_ERR_ARG;
   if (req->tangent_mode == MM_TANGENT_FD && req->nload < 7) return MM_ERR_ARG;
   for (int k = 0; k < req->nload; k++)
@@ -506,10 +531,21 @@ extern "C" int mm_engine_submit(mm_engine *e, const mm_request *req)
       if (!std::isfinite(req->strain[k][i])) return MM_ERR_ARG;
   if (req->tangent_mode == MM_TANGENT_UNIT)
     for (int i = 0; i < 6; i++)
-      if (req->strain[i][i] == 0.0) return MM_ERR_ARG;
+      for (int j = 0; j < MM_NVOI; j++)
+        if ((j == i) != (req->strain[i][j] != 0.0)) return MM_ERR_ARG;  // load i must be a e_i, a != 0
   if (req->tangent_mode == MM_TANGENT_FD)
     for (int i = 0; i < 6; i++)
-      if (req->strain[i + 1][i] == req->strain[0][i]) return MM_ERR_ARG;
+      for (int j = 0; j < MM_NVOI; j++)
+        if ((j == i) != (req->strain[i + 1][j] != req->strain[0][j])) return MM_ERR_ARG;  // eps + d e_i
+  if (req->warm_start != 0 && req->warm_start != 1) return MM_ERR_ARG;
+  return MM_OK;
+}
+
+extern "C" int mm_engine_submit(mm_engine *e, const mm_request *req)
+{
+  if (!e || !req) return MM_ERR_ARG;
+  const int vrc = mm_request_validate(req, e->K);
+  if (vrc) return vrc;
   int nrow;
   long long nslab, nelm;
   footprint(e, req->rve, &nrow, &nslab, &nelm);
@@ -549,6 +585,82 @@ static bool evict_one(mm_engine *e)
   return true;
 }
 
+// Warm-start coefficients.  The kept displacements x_old[j] solve the loads
+// old[j]; for linear materials the solution is linear in the load, so when
+// six of the old loads span the strain space, a new load e = sum_j w_j old[j]
+// starts from sum_j w_j x_old[j], which is already converged up to the old
+// solves' tolerance.  Otherwise load k starts from x_old[k] plus the affine
+// field of the strain increment e_k - old[k].  W[k][j] are the weights and
+// D[k] the strain whose affine field is added (zero up to rounding in the
+// first case).
+static void warm_coefficients(const double (*old)[MM_NVOI], int nold, const double (*nw)[MM_NVOI], int nnew,
+                              double (*W)[MM_MAX_RHS], double (*D)[MM_NVOI])
+{
+  for (int k = 0; k < MM_MAX_RHS; k++) {
+    for (int j = 0; j < MM_MAX_RHS; j++) W[k][j] = 0.0;
+    for (int i = 0; i < MM_NVOI; i++) D[k][i] = 0.0;
+  }
+  // old loads that span the strain space: modified Gram-Schmidt, a load is
+  // kept when its component outside the span of the kept ones is more than
+  // 1e-6 of its length
+  int pick[MM_NVOI], np = 0;
+  double Q[MM_NVOI][MM_NVOI];
+  for (int j = 0; j < nold && np < MM_NVOI; j++) {
+    double v[MM_NVOI], len0 = 0.0, len = 0.0;
+    for (int i = 0; i < MM_NVOI; i++) {
+      v[i] = old[j][i];
+      len0 += v[i] * v[i];
+    }
+    for (int q = 0; q < np; q++) {
+      double dot = 0.0;
+      for (int i = 0; i < MM_NVOI; i++) dot += Q[q][i] * v[i];
+      for (int i = 0; i < MM_NVOI; i++) v[i] -= dot * Q[q][i];
+    }
+    for (int i = 0; i < MM_NVOI; i++) len += v[i] * v[i];
+    if (len0 > 0.0 && len > 1e-12 * len0) {
+      const double inv = 1.0 / std::sqrt(len);
+      for (int i = 0; i < MM_NVOI; i++) Q[np][i] = v[i] * inv;
+      pick[np++] = j;
+    }
+  }
+  for (int k = 0; k < nnew; k++) {
+    if (np == MM_NVOI) {
+      // solve M w = e, column c of M the picked load pick[c] (partial pivoting)
+      double A[MM_NVOI][MM_NVOI + 1];
+      for (int r = 0; r < MM_NVOI; r++) {
+        for (int c = 0; c < MM_NVOI; c++) A[r][c] = old[pick[c]][r];
+        A[r][MM_NVOI] = nw[k][r];
+      }
+      for (int c = 0; c < MM_NVOI; c++) {
+        int p = c;
+        for (int r = c + 1; r < MM_NVOI; r++)
+          if (std::fabs(A[r][c]) > std::fabs(A[p][c])) p = r;
+        if (p != c)
+          for (int cc = 0; cc <= MM_NVOI; cc++) std::swap(A[p][cc], A[c][cc]);
+        for (int r = c + 1; r < MM_NVOI; r++) {
+          const double f = A[r][c] / A[c][c];
+          for (int cc = c; cc <= MM_NVOI; cc++) A[r][cc] -= f * A[c][cc];
+        }
+      }
+      double w[MM_NVOI];
+      for (int c = MM_NVOI - 1; c >= 0; c--) {
+        double t = A[c][MM_NVOI];
+        for (int cc = c + 1; cc < MM_NVOI; cc++) t -= A[c][cc] * w[cc];
+        w[c] = t / A[c][c];
+      }
+      for (int c = 0; c < MM_NVOI; c++) W[k][pick[c]] = w[c];
+      for (int i = 0; i < MM_NVOI; i++) {
+        double s = nw[k][i];
+        for (int c = 0; c < MM_NVOI; c++) s -= w[c] * old[pick[c]][i];
+        D[k][i] = s;
+      }
+    } else {
+      if (k < nold) W[k][k] = 1.0;
+      for (int i = 0; i < MM_NVOI; i++) D[k][i] = nw[k][i] - (k < nold ? old[k][i] : 0.0);
+    }
+  }
+}
+
 static void fill_state(const mm_engine *e, const HostSlot &h, RveDev *st, int warm)
 {
   memset(st, 0, sizeof(*st));
@@ -585,6 +697,9 @@ static void fill_state(const mm_engine *e, const HostSlot &h, RveDev *st, int wa
   st->affine = (e->cfg.initial_guess == MM_GUESS_AFFINE) ? 1 : 0;
   for (int k = 0; k < h.req.nload; k++)
     for (int i = 0; i < MM_NVOI; i++) st->load[k][i] = h.req.strain[k][i];
+  if (warm)
+    warm_coefficients(h.prev_load, std::min(h.prev_nload, e->K), h.req.strain, h.req.nload, st->wcoef,
+                      st->dload);
   for (int c = 0; c < 3; c++) st->st[c].phase = PH_RES;
 }
 
@@ -604,6 +719,8 @@ static int admit(mm_engine *e)
           HostSlot &h = e->slots[it->second];
           if (h.state == SL_ACTIVE) break;  // same key still in flight: keep FIFO order
           if (h.state == SL_CACHED && same_geometry(h.req.rve, req.rve)) {
+            memcpy(h.prev_load, h.req.strain, sizeof(h.prev_load));
+            h.prev_nload = h.req.nload;
             h.req = req;
             h.state = SL_ACTIVE;
             h.t_submit = qd.t_submit;
@@ -790,6 +907,22 @@ static int queue_steps(mm_engine *e, int grid0, int grid1)
   const int nk = 3 * e->cfg.steps_per_launch;
   const bool f64 = e->fp64, dm = e->dict_mode;
   int rc = 0;
+  if (e->cfg.profile_step_kernel && grid1 == 0) {
+    const int ns = e->cfg.steps_per_launch;
+    while ((int)e->prof_ev.size() < 2 * ns) {
+      hipEvent_t ev;
+      MM_HIP(hipEventCreate(&ev));
+      e->prof_ev.push_back(ev);
+    }
+    for (int i = 0; i < ns; i++) {
+      MM_HIP(hipEventRecord(e->prof_ev[2 * i], e->stream));
+      if ((rc = eng_launch_steps(f64, e->K, dm, P0, sp, grid0, 0, 1, e->stream))) return rc;
+      MM_HIP(hipEventRecord(e->prof_ev[2 * i + 1], e->stream));
+      if ((rc = eng_launch_steps(f64, e->K, dm, P0, sp, grid0, 1, 2, e->stream))) return rc;
+    }
+    e->prof_pending = ns;
+    return 0;
+  }
   if (grid1 == 0) return eng_launch_steps(f64, e->K, dm, P0, sp, grid0, 0, nk, e->stream);
   int first = 0;
   if (grid0 > 0) {
@@ -885,6 +1018,12 @@ extern "C" int mm_engine_step(mm_engine *e)
     if (rc) return e->last_error = rc;
   }
   MM_HIP(hipStreamSynchronize(e->stream));
+  for (int i = 0; i < e->prof_pending; i++) {
+    float ms = 0.0f;
+    MM_HIP(hipEventElapsedTime(&ms, e->prof_ev[2 * i], e->prof_ev[2 * i + 1]));
+    e->prof_ms.push_back(ms);
+  }
+  e->prof_pending = 0;
   e->nsteps++;
   retire(e);
   return 0;
@@ -1000,6 +1139,17 @@ extern "C" int mm_engine_info_get(mm_engine *e, mm_engine_info *info)
                          : 0;
   info->slab_nodes = e->SN;
   info->spmv_ms = e->spmv_ms;
+  return 0;
+}
+
+extern "C" int mm_engine_step_kernel_times(mm_engine *e, double *ms, int max, int *n)
+{
+  if (!e || !n || (max > 0 && !ms)) return MM_ERR_ARG;
+  std::lock_guard<std::mutex> lk(e->step_mu);
+  const int c = std::min<int>(max, (int)e->prof_ms.size());
+  for (int i = 0; i < c; i++) ms[i] = e->prof_ms[i];
+  *n = c;
+  e->prof_ms.clear();
   return 0;
 }
 
@@ -1238,6 +1388,93 @@ extern "C" int mm_hbm_copy_probe(int device, size_t bytes, int reps, double *avg
     }
   if (hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
       hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
+    rc = MM_ERR_HIP;
+    goto done;
+  }
+  *avg_ms = ms / reps;
+done:
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  if (s) (void)hipStreamDestroy(s);
+  if (a) (void)hipFree(a);
+  if (b) (void)hipFree(b);
+  return rc;
+}
+
+// ---------------------------------------------------------------------------
+// Hand-written streaming probes: the practical HBM ceiling of this device for
+// the two access patterns of the solver kernels.  Grid-stride loops over
+// 16-byte non-temporal accesses, 8 workgroups of 256 lanes per CU.
+namespace {
+
+typedef float stream_f4 __attribute__((ext_vector_type(4)));
+
+// dst = src (reads and writes `n` 16-byte words)
+__global__ __launch_bounds__(256) void k_stream_copy(const stream_f4 *__restrict__ src, stream_f4 *__restrict__ dst,
+                                                     long long n)
+{
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+    __builtin_nontemporal_store(__builtin_nontemporal_load(src + i), dst + i);
+}
+
+// reads `n` 16-byte words; one float per workgroup is written so that the
+// loads are not dead
+__global__ __launch_bounds__(256) void k_stream_read(const stream_f4 *__restrict__ src, float *__restrict__ out,
+                                                     long long n)
+{
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  float acc = 0.0f;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    const stream_f4 v = __builtin_nontemporal_load(src + i);
+    acc += (v.x + v.y) + (v.z + v.w);
+  }
+  __shared__ float red[256];
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[blockIdx.x] = red[0];
+}
+
+}  // namespace
+
+extern "C" int mm_hbm_stream_probe(int device, size_t bytes, int reps, int mode, double *avg_ms)
+{
+  if (bytes < 16 || reps < 1 || !avg_ms || (mode != 0 && mode != 1)) return MM_ERR_ARG;
+  MM_HIP(hipSetDevice(device));
+  int ncu = 0;
+  MM_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, device));
+  const long long n = (long long)(bytes / 16);
+  const int grid = 8 * (ncu > 0 ? ncu : 256);
+  void *a = nullptr, *b = nullptr;
+  hipStream_t s = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  int rc = 0;
+  float ms = 0.0f;
+  const size_t bbytes = (mode == 0) ? (size_t)n * 16 : (size_t)grid * sizeof(float);
+  if (hipMalloc(&a, (size_t)n * 16) != hipSuccess || hipMalloc(&b, bbytes) != hipSuccess ||
+      hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess || hipEventCreate(&e0) != hipSuccess ||
+      hipEventCreate(&e1) != hipSuccess) {
+    rc = MM_ERR_HIP;
+    goto done;
+  }
+  if (hipMemsetAsync(a, 0, (size_t)n * 16, s) != hipSuccess) {
+    rc = MM_ERR_HIP;
+    goto done;
+  }
+  for (int i = 0; i <= reps; i++) {  // launch 0 is the warm-up
+    if (i == 1 && hipEventRecord(e0, s) != hipSuccess) {
+      rc = MM_ERR_HIP;
+      goto done;
+    }
+    if (mode == 0) k_stream_copy<<<grid, 256, 0, s>>>((const stream_f4 *)a, (stream_f4 *)b, n);
+    else k_stream_read<<<grid, 256, 0, s>>>((const stream_f4 *)a, (float *)b, n);
+  }
+  if (hipGetLastError() != hipSuccess || hipEventRecord(e1, s) != hipSuccess ||
+      hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
     rc = MM_ERR_HIP;
     goto done;
   }

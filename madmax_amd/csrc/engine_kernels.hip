@@ -220,6 +220,16 @@ __global__ void k_phase(int nblk, const int2 *__restrict__ work, const RveDev *_
   elem_mat[R.elem_base + e] = m;
 }
 
+// Component d of the affine displacement u = E X of Voigt strain s, shears
+// halved (micro/src/homogenize.c:335-342).
+__device__ __forceinline__ double affine_u(const double *s, const double (&X)[3], int d)
+{
+#pragma clang fp contract(off)
+  if (d == 0) return s[0] * X[0] + s[3] / 2 * X[1] + s[5] / 2 * X[2];
+  if (d == 1) return s[3] / 2 * X[0] + s[1] * X[1] + s[4] / 2 * X[2];
+  return s[5] / 2 * X[0] + s[4] / 2 * X[1] + s[2] * X[2];
+}
+
 // ELL assembly: one lane per node, writing the three rows of the node.  Every
 // slot value is the sum of the Ke entries of the elements sharing the two
 // nodes, in ascending element id, the first stored and the rest added
@@ -337,18 +347,25 @@ __global__ void k_assemble(int nblk, const int *__restrict__ slab_list, const in
     for (int d = 0; d < 3; d++) dinv[grow0 + d] = (TV)(1.0 / diag[d]);
   }
   // initial displacement for every load case: affine on the boundary; inside
-  // zero, the affine field, or (warm start) the kept values
+  // zero, the affine field, or (warm start) the kept displacements combined
+  // as sum_j wcoef[k][j] x_old[j] plus the affine field of dload[k]
   const double X[3] = {ix * R.hx, iy * R.hy, iz * R.hz};
+  double xo[3][K];
+  if (R.warm && !bc)
+    for (int d = 0; d < 3; d++)
+      for (int k = 0; k < K; k++) xo[d][k] = x64[xb + (long long)k * RS + d];
   for (int d = 0; d < 3; d++)
     for (int k = 0; k < K; k++) {
-      if (k < R.nload && (bc || (R.affine && !R.warm))) {
-        const double *s = R.load[k];
-        double u;
-        if (d == 0) u = s[0] * X[0] + s[3] / 2 * X[1] + s[5] / 2 * X[2];
-        else if (d == 1) u = s[3] / 2 * X[0] + s[1] * X[1] + s[4] / 2 * X[2];
-        else u = s[5] / 2 * X[0] + s[4] / 2 * X[1] + s[2] * X[2];
+      if (k >= R.nload) {
+        x64[xb + (long long)k * RS + d] = 0.0;
+      } else if (bc || (R.affine && !R.warm)) {
+        x64[xb + (long long)k * RS + d] = affine_u(R.load[k], X, d);
+      } else if (R.warm) {
+        double u = affine_u(R.dload[k], X, d);
+        for (int j = 0; j < K; j++)
+          if (R.wcoef[k][j] != 0.0) u = u + R.wcoef[k][j] * xo[d][j];
         x64[xb + (long long)k * RS + d] = u;
-      } else if (!(R.warm && k < R.nload)) {
+      } else {
         x64[xb + (long long)k * RS + d] = 0.0;
       }
     }

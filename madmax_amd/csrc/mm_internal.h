@@ -115,6 +115,10 @@ struct RveDev {
   double fx[MM_MAX_FIBERS], fy[MM_MAX_FIBERS], fr[MM_MAX_FIBERS];
   double young[kNmat], poisson[kNmat];
   double load[MM_MAX_RHS][MM_NVOI];
+  // warm start (warm != 0): interior displacement of load k starts at
+  // sum_j wcoef[k][j] x_old[j] + E(dload[k]) X, x_old the kept displacements
+  double wcoef[MM_MAX_RHS][MM_MAX_RHS];
+  double dload[MM_MAX_RHS][MM_NVOI];
   SolveState st[3];
 };
 

@@ -86,7 +86,7 @@ class Engine:
                  slab_nodes=0, rtol=1e-10, abs_tol=0.0, inner_rtol=1e-5, max_inner_its=20000,
                  max_sweeps=8, steps_per_launch=16, use_graph=True, sync_every_step=False,
                  device=0, ell_storage=_lib.MM_ELL_DICT, initial_guess=_lib.MM_GUESS_AFFINE,
-                 solver_streams=2):
+                 solver_streams=2, profile_step_kernel=False):
         L = lib()
         c = EngineConfig()
         check(L.mm_engine_config_default(C.byref(c)), "mm_engine_config_default")
@@ -107,6 +107,7 @@ class Engine:
         c.ell_storage = int(ell_storage)
         c.initial_guess = int(initial_guess)
         c.solver_streams = int(solver_streams)
+        c.profile_step_kernel = int(bool(profile_step_kernel))
         self.cfg = c
         self.nrhs = nrhs
         h = C.c_void_p()
@@ -152,7 +153,9 @@ class Engine:
         return lib().mm_engine_now(self.h)
 
     def poll(self, max_results=1024):
-        buf = (Result * max_results)()
+        buf = getattr(self, "_buf", None)
+        if buf is None or len(buf) < max_results:
+            buf = self._buf = (Result * max_results)()
         n = C.c_int()
         check(lib().mm_engine_poll(self.h, buf, max_results, C.byref(n)), "mm_engine_poll")
         out = []
@@ -165,6 +168,14 @@ class Engine:
         i = EngineInfo()
         check(lib().mm_engine_info_get(self.h, C.byref(i)), "mm_engine_info_get")
         return {f: getattr(i, f) for f, _ in EngineInfo._fields_}
+
+    def step_kernel_times(self, max_n=1 << 16):
+        """Durations (ms) of the step-kernel launches recorded since the last
+        call (engine created with profile_step_kernel=True)."""
+        buf = np.zeros(max_n)
+        n = C.c_int()
+        check(lib().mm_engine_step_kernel_times(self.h, _dp(buf), max_n, C.byref(n)), "step_kernel_times")
+        return buf[: n.value].copy()
 
     def rve_sizes(self, key):
         nr, ne = C.c_int(), C.c_int()

@@ -111,3 +111,45 @@ def test_argument_errors_without_gpu():
     assert L.ell_device_create(C.byref(e.m), 0, C.byref(h)) == 1
     e = Ell.from_arrays(np.array([[0, -2]], dtype=np.int32), np.array([[1.0, 2.0]]), ncol=2)
     assert L.ell_device_create(C.byref(e.m), 0, C.byref(h)) == 1
+
+
+def test_request_validation_refuses_loads_the_tangent_cannot_divide_by():
+    """mm_request_validate (run by mm_engine_submit before anything reaches the
+    GPU): a UNIT tangent divides the stress of load i by its component i, an
+    FD tangent divides by the difference of loads i+1 and 0 in component i
+    (micro/src/homogenize.c:59-94), so loads of any other shape are refused
+    instead of producing a wrong tangent with status MM_OK."""
+    from madmax_amd.engine import fd_loads, make_request, rve_spec, unit_loads
+    L = _lib.lib()
+    spec = rve_spec(4, [(0.5, 0.5, 0.3)])
+
+    def rc(loads, mode, nrhs=None):
+        rq = make_request(1, spec, loads, mode)
+        return L.mm_request_validate(C.byref(rq), nrhs or len(loads))
+
+    unit = unit_loads()
+    assert rc(unit, _lib.MM_TANGENT_UNIT) == 0
+    assert rc(unit + [np.array([1e-3, 2e-4, 0, 0, 0, 0])], _lib.MM_TANGENT_UNIT) == 0
+    assert rc([2.5 * u for u in unit], _lib.MM_TANGENT_UNIT) == 0  # any a_i != 0
+    bad = [u.copy() for u in unit]
+    bad[3][0] = 1e-4  # load 3 has an off-diagonal component
+    assert rc(bad, _lib.MM_TANGENT_UNIT) == _lib.MM_ERR_ARG
+    bad = [u.copy() for u in unit]
+    bad[5][5] = 0.0  # zero diagonal
+    assert rc(bad, _lib.MM_TANGENT_UNIT) == _lib.MM_ERR_ARG
+    eps = np.array([1e-3, -2e-4, 0.0, 3e-4, 0.0, 1e-4])
+    fd = fd_loads(eps)
+    assert rc(fd, _lib.MM_TANGENT_FD) == 0
+    bad = [l.copy() for l in fd]
+    bad[2][4] += 1e-6  # load 2 = eps + d e_1 must not move component 4
+    assert rc(bad, _lib.MM_TANGENT_FD) == _lib.MM_ERR_ARG
+    bad = [l.copy() for l in fd]
+    bad[4][3] = bad[0][3]  # no step in its own component
+    assert rc(bad, _lib.MM_TANGENT_FD) == _lib.MM_ERR_ARG
+    assert rc(unit[:5], _lib.MM_TANGENT_UNIT) == _lib.MM_ERR_ARG  # too few loads
+    assert rc(unit, _lib.MM_TANGENT_UNIT, nrhs=5) == _lib.MM_ERR_ARG  # more loads than the block
+    assert rc(unit, 9) == _lib.MM_ERR_ARG  # unknown tangent mode
+    assert rc([eps], _lib.MM_TANGENT_NONE) == 0
+    nan = eps.copy()
+    nan[2] = float("nan")
+    assert rc([nan], _lib.MM_TANGENT_NONE) == _lib.MM_ERR_ARG

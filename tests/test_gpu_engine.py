@@ -347,3 +347,59 @@ def test_lru_eviction_keeps_results_exact():
         again = eng.poll()[0]
     assert all(r["status"] == 0 for r in first.values())
     assert np.array_equal(again["c_tangent"], first[0]["c_tangent"])
+
+
+def test_warm_start_superposes_kept_solutions():
+    """A warm start on a resident linear RVE combines the kept displacements:
+    when six kept load cases span the strain space (a unit-load tangent
+    block), the next strain starts from the superposed solution and needs
+    far fewer PCG iterations than a cold start; with a single kept load it
+    starts from the kept solution plus the affine field of the strain
+    increment.  Every answer still meets the oracle bar."""
+    spec = configs.c2_spec(2, 5, n=12)
+    ref, _ = oracle_rve(spec)
+    s1 = np.array([4e-4, -2e-4, 1e-4, 3e-4, -1e-4, 2e-4])
+    s2 = 1.05 * s1 + np.array([0, 1e-5, 0, 0, 2e-5, 0])
+    eye = unit_loads()
+    with Engine(nrhs=7, hbm_budget=GB, max_rves=4) as eng:
+        assert eng.submit(1, spec, eye + [s1], MM_TANGENT_UNIT, warm_key=9) == 0
+        eng.drain()
+        first = eng.poll()[0]
+        out = {}
+        for rid, warm in ((2, 1), (3, 0)):
+            assert eng.submit(rid, spec, eye + [s2], MM_TANGENT_UNIT, warm_key=9, warm_start=warm) == 0
+            eng.drain()
+            out[warm] = eng.poll()[0]
+            if warm:  # restore the kept solutions of s1 for the cold comparison
+                assert eng.submit(10 + rid, spec, eye + [s1], MM_TANGENT_UNIT, warm_key=9) == 0
+                eng.drain()
+                eng.poll()
+    c_ref, _ = ref.c_tangent_unit(1e-3, 1e-12)
+    st_ref = ref.homog(s2, rtol=1e-12)["stress_ave"]
+    for r in (first, out[0], out[1]):
+        assert r["status"] == 0
+        assert normwise_rel(r["c_tangent"], c_ref) <= TOL
+    for r in (out[0], out[1]):
+        assert normwise_rel(r["stress"][6], st_ref) <= TOL
+    print("superposed warm its", out[1]["pcg_its"], "cold its", out[0]["pcg_its"], flush=True)
+    assert out[1]["pcg_its"] < out[0]["pcg_its"] // 2
+
+    # one kept load: start from x_old + E(s2 - s1) X
+    with Engine(nrhs=1, hbm_budget=GB, max_rves=4) as eng:
+        assert eng.submit(1, spec, [s1], MM_TANGENT_NONE, warm_key=5) == 0
+        eng.drain()
+        eng.poll()
+        res = {}
+        for rid, warm in ((2, 1), (3, 0)):
+            assert eng.submit(rid, spec, [s2], MM_TANGENT_NONE, warm_key=5, warm_start=warm) == 0
+            eng.drain()
+            res[warm] = eng.poll()[0]
+            if warm:
+                assert eng.submit(10 + rid, spec, [s1], MM_TANGENT_NONE, warm_key=5) == 0
+                eng.drain()
+                eng.poll()
+    for r in res.values():
+        assert r["status"] == 0
+        assert normwise_rel(r["stress"][0], st_ref) <= TOL
+    print("increment warm its", res[1]["pcg_its"], "cold its", res[0]["pcg_its"], flush=True)
+    assert res[1]["pcg_its"] < res[0]["pcg_its"]
