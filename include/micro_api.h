@@ -39,6 +39,11 @@ extern "C" {
 #define MM_ERR_STATE 7
 #define MM_REJECTED 8 /* larger than the whole HBM budget */
 
+/* boundary conditions of the RVE (the values of micro/inc/micro.h:26) */
+#define MICRO_BC_USTRAIN 1 /* u = E X on the six faces */
+#define MICRO_BC_USTRESS 2 /* only sized by the reference (micro/src/alloc.c:18-19): refused */
+#define MICRO_BC_PER_LM 3  /* periodic: corners u = E X, u+ - u- = E (X+ - X-) on opposite faces */
+
 /* solver choice for the process-global micro */
 #define MICRO_SOLVER_PCG 0       /* FP64 Jacobi-PCG (PETSc cg + jacobi semantics) */
 #define MICRO_SOLVER_PCG_MIXED 1 /* FP32 Jacobi-PCG inside FP64 iterative refinement */
@@ -68,6 +73,13 @@ typedef struct {
   double nl_min_norm;  /* absolute residual threshold (micro/src/init.c:8, default 1e-4) */
   double rho;          /* density answered to ACTION_MICRO_CALC_RHO (micro/src/init.c:9) */
   int device;
+  int linear_materials; /* 1 (default): the tangent is computed once at init and
+                           stresses come from it (micro/src/homogenize.c:3-21);
+                           0: flags.linear_materials == false, every request
+                           runs homog_fe2's Newton loop with the Jacobian
+                           re-assembled element by element on the GPU at every
+                           iteration (micro/src/homogenize.c:23-49, 175-225) */
+  int fe2_bc;           /* MICRO_BC_USTRAIN (default) or MICRO_BC_PER_LM */
 } micro_config;
 
 int micro_config_default(micro_config *cfg);
@@ -114,14 +126,28 @@ typedef struct {
   int sweeps;       /* Newton / refinement sweeps of the last solve */
   double residual;  /* final FP64 residual norm (max over RHS) */
   double seconds;   /* wall time of the last solve */
+  int assemblies;   /* Jacobian assemblies since micro_init (element path) */
+  int pad_;
 } micro_stats;
 int micro_get_stats(micro_stats *st);
 int micro_get_sizes(int *nrow, int *nnz, int *nelm);
 /* assembled Jacobian in the reference's row-major ELL layout */
 int micro_get_ell(int *cols, double *vals);
 int micro_get_elem_type(int *elem_type);
-/* displacement of the last homog_fe2 call (length nrow) */
+/* displacement of the last homog_fe2 call (length nrow; with
+ * MICRO_BC_PER_LM the periodic fluctuation on the nex x ney x nez nodes of
+ * the torus, u = E X + x) */
 int micro_get_displacement(double *x);
+/* assembly_res_ell (micro/src/assembly.c:33-82) on the GPU for a given
+ * displacement x (length nrow): r = sum_e sum_gp B^T sigma w with the
+ * boundary dofs zeroed, and |r|_2.  With MICRO_BC_PER_LM, x is the periodic
+ * fluctuation and strain the macro strain (NULL: zero) of the affine part. */
+int micro_residual(const double *strain, const double *x, double *r, double *norm);
+/* Writes <name>.pvtu and <name>_0.vtu for the last homog_fe2 call like
+ * micro_pvtu (micro/src/output.c:43-188): node positions, hex8 cells,
+ * displacement and residual per node, part, strain, stress (element averages
+ * over the gauss points) and elem_type per cell. */
+int micro_write_vtu(const char *name);
 
 /* ---------------------------------------------------------------------------
  * Batched / serving engine: many RVEs of uneven size resident in one HBM

@@ -18,6 +18,7 @@ HOMOGENIZE_DELTA_STRAIN = 0.005
 MM_OK, MM_ERR_ARG, MM_ERR_ALLOC, MM_ERR_FULL = 0, 1, 2, 3
 MM_NOT_CONVERGED, MM_ERR_HIP, MM_ERR_STATE, MM_REJECTED = 5, 6, 7, 8
 MICRO_SOLVER_PCG, MICRO_SOLVER_PCG_MIXED = 0, 1
+MICRO_BC_USTRAIN, MICRO_BC_USTRESS, MICRO_BC_PER_LM = 1, 2, 3
 MM_PREC_FP64, MM_PREC_MIXED = 0, 1
 MM_ELL_FULL, MM_ELL_DICT = 0, 1
 MM_GUESS_ZERO, MM_GUESS_AFFINE = 0, 1
@@ -51,7 +52,7 @@ class RveSpec(C.Structure):
 class MicroConfig(C.Structure):
     _fields_ = [("rve", RveSpec), ("solver", C.c_int), ("rtol", C.c_double),
                 ("nl_max_its", C.c_int), ("nl_min_norm", C.c_double), ("rho", C.c_double),
-                ("device", C.c_int)]
+                ("device", C.c_int), ("linear_materials", C.c_int), ("fe2_bc", C.c_int)]
 
 
 class MicroMessage(C.Structure):
@@ -67,7 +68,7 @@ ACTION_MICRO_CALC_RHO, ACTION_MICRO_END = 3, 4
 
 class MicroStats(C.Structure):
     _fields_ = [("pcg_its", C.c_int), ("sweeps", C.c_int), ("residual", C.c_double),
-                ("seconds", C.c_double)]
+                ("seconds", C.c_double), ("assemblies", C.c_int), ("pad_", C.c_int)]
 
 
 class EngineConfig(C.Structure):
@@ -142,6 +143,8 @@ SIGNATURES = {
     "micro_get_ell": (C.c_int, [ip, dp]),
     "micro_get_elem_type": (C.c_int, [ip]),
     "micro_get_displacement": (C.c_int, [dp]),
+    "micro_write_vtu": (C.c_int, [C.c_char_p]),
+    "micro_residual": (C.c_int, [dp, dp, dp, dp]),
     "mm_engine_config_default": (C.c_int, [C.POINTER(EngineConfig)]),
     "mm_request_validate": (C.c_int, [C.POINTER(Request), C.c_int]),
     "mm_engine_create": (C.c_int, [C.POINTER(EngineConfig), C.POINTER(C.c_void_p)]),
@@ -162,6 +165,18 @@ SIGNATURES = {
     "mm_engine_time_spmv": (C.c_int, [C.c_void_p, C.c_int, dp, dp]),
     "mm_engine_time_step": (C.c_int, [C.c_void_p, C.c_int, dp, dp]),
     "mm_hbm_copy_probe": (C.c_int, [C.c_int, C.c_size_t, C.c_int, dp]),
+    # include/macro_api.h
+    "mm_macro_create": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double, dp, C.c_int,
+                                  C.POINTER(C.c_void_p)]),
+    "mm_macro_destroy": (C.c_int, [C.c_void_p]),
+    "mm_macro_sizes": (C.c_int, [C.c_void_p, ip, ip]),
+    "mm_macro_set_dirichlet": (C.c_int, [C.c_void_p, C.c_int, ip]),
+    "mm_macro_set_u": (C.c_int, [C.c_void_p, dp]),
+    "mm_macro_strain": (C.c_int, [C.c_void_p, dp]),
+    "mm_macro_assemble_b": (C.c_int, [C.c_void_p, dp, dp, dp]),
+    "mm_macro_assemble_a": (C.c_int, [C.c_void_p, dp]),
+    "mm_macro_get_ell": (C.c_int, [C.c_void_p, ip, dp]),
+    "mm_macro_solve": (C.c_int, [C.c_void_p, dp, dp, C.c_double, ip]),
     "mm_hbm_stream_probe": (C.c_int, [C.c_int, C.c_size_t, C.c_int, C.c_int, dp]),
 }
 

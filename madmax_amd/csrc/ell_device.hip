@@ -17,9 +17,11 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 #include "ell.h"
+#include "fe.h"
 #include "mm_device.h"
 #include "mm_internal.h"
 
@@ -272,10 +274,24 @@ __global__ void k_pcg_update(int nrow, int S, const double *__restrict__ dinv,
   }
 }
 
+// x = 0 (a kernel, not a memset node, so that the solve can sit in a graph)
+__global__ void k_zero(double *__restrict__ x, int n)
+{
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) x[i] = 0.0;
+}
+
+// PCG work vectors of one solve
+struct PcgVecs {
+  double *r, *z, *pA, *pB, *q, *part;
+  IterState *st;
+};
+
 }  // namespace
 
 struct ell_device_ {
   int nrow, ncol, nnz, S, nslab;
+  int device;  // HIP device the mirror lives on
   int *cols;
   double *vals, *dinv;
   double *x, *b, *r, *z, *pA, *pB, *q, *part;
@@ -354,6 +370,10 @@ extern "C" int ell_device_create(const ell_matrix *m, int slab_rows, ell_device 
   d->nnz = m->nnz;
   d->S = S;
   d->nslab = (m->nrow + S - 1) / S;
+  if (hipGetDevice(&d->device) != hipSuccess) {
+    free(d);
+    return MM_ERR_HIP;
+  }
   const size_t slots = (size_t)d->nslab * S * d->nnz;
   const size_t nv = (size_t)(d->nslab ? d->nslab : 1) * S;
   const size_t nvc = (size_t)(m->ncol > (int)nv ? m->ncol : nv);
@@ -457,20 +477,20 @@ extern "C" int ell_device_mvp(ell_device *d, const double *x, double *y)
   return 0;
 }
 
-static int set_state(ell_device *d, int max_its, double rtol, double min_err)
+static int set_state(ell_device *d, IterState *st, int max_its, double rtol, double min_err)
 {
   IterState h;
   memset(&h, 0, sizeof(h));
   h.max_its = max_its;
   h.rtol = rtol;
   h.min_err = min_err;
-  MM_HIP(hipMemcpyAsync(d->st, &h, sizeof(h), hipMemcpyHostToDevice, d->stream));
+  MM_HIP(hipMemcpyAsync(st, &h, sizeof(h), hipMemcpyHostToDevice, d->stream));
   return 0;
 }
 
-static int get_state(ell_device *d, IterState *h)
+static int get_state(ell_device *d, const IterState *st, IterState *h)
 {
-  MM_HIP(hipMemcpyAsync(h, d->st, sizeof(*h), hipMemcpyDeviceToHost, d->stream));
+  MM_HIP(hipMemcpyAsync(h, st, sizeof(*h), hipMemcpyDeviceToHost, d->stream));
   MM_HIP(hipStreamSynchronize(d->stream));
   return 0;
 }
@@ -480,7 +500,7 @@ extern "C" int ell_device_solve_jacobi(ell_device *d, ell_solver *solver, const 
   if (!d || !b || !x) return 1;
   if (d->nrow != d->ncol) return 1;
   if (d->nrow == 0) return 0;
-  int rc = set_state(d, MAX_ITS, 0.0, MIN_ERR);
+  int rc = set_state(d, d->st, MAX_ITS, 0.0, MIN_ERR);
   if (rc) return rc;
   MM_HIP(hipMemcpyAsync(d->x, x, d->nrow * sizeof(double), hipMemcpyHostToDevice, d->stream));
   MM_HIP(hipMemcpyAsync(d->b, b, d->nrow * sizeof(double), hipMemcpyHostToDevice, d->stream));
@@ -493,7 +513,7 @@ extern "C" int ell_device_solve_jacobi(ell_device *d, ell_solver *solver, const 
       k_jac_upd<<<d->nslab, d->S, 0, d->stream>>>(d->nrow, d->S, d->dinv, d->r, d->x, d->st);
     }
     MM_HIP(hipGetLastError());
-    rc = get_state(d, &h);
+    rc = get_state(d, d->st, &h);
     if (rc) return rc;
     if (h.done) break;
   }
@@ -506,41 +526,40 @@ extern "C" int ell_device_solve_jacobi(ell_device *d, ell_solver *solver, const 
   return 0;
 }
 
-extern "C" int ell_device_solve_pcg(ell_device *d, ell_solver *solver, const double *b, double *x)
+// Jacobi-PCG on the device mirror with b and x (initial guess in, solution
+// out) in device memory and the work vectors V; the host polls the
+// iteration state every 32 iterations.  Returns 0 when converged, 5 when
+// max_its was reached.
+static int pcg_run(ell_device *d, const PcgVecs &V, ell_solver *solver, const double *b, double *x)
 {
-  if (!d || !b || !x) return 1;
-  if (d->nrow != d->ncol) return 1;
-  if (d->nrow == 0) return 0;
-  int max_its = (solver && solver->max_its > 0) ? solver->max_its : MAX_ITS;
-  double rtol = (solver && solver->min_err > 0) ? solver->min_err : MIN_ERR;
-  int rc = set_state(d, max_its, rtol, 0.0);
+  const int max_its = (solver && solver->max_its > 0) ? solver->max_its : MAX_ITS;
+  const double rtol = (solver && solver->min_err > 0) ? solver->min_err : MIN_ERR;
+  int rc = set_state(d, V.st, max_its, rtol, 0.0);
   if (rc) return rc;
-  MM_HIP(hipMemcpyAsync(d->x, x, d->nrow * sizeof(double), hipMemcpyHostToDevice, d->stream));
-  MM_HIP(hipMemcpyAsync(d->b, b, d->nrow * sizeof(double), hipMemcpyHostToDevice, d->stream));
-  k_pcg_init<<<d->nslab, d->S, 0, d->stream>>>(d->cols, d->vals, d->nrow, d->nnz, d->S, d->x, d->b,
-                                               d->dinv, d->r, d->z, d->pA, d->pB, d->part);
-  k_reduce<<<1, 256, 0, d->stream>>>(d->part, d->nslab, RED_INIT, d->st);
+  k_pcg_init<<<d->nslab, d->S, 0, d->stream>>>(d->cols, d->vals, d->nrow, d->nnz, d->S, x, b, d->dinv, V.r,
+                                               V.z, V.pA, V.pB, V.part);
+  k_reduce<<<1, 256, 0, d->stream>>>(V.part, d->nslab, RED_INIT, V.st);
   MM_HIP(hipGetLastError());
   IterState h;
-  rc = get_state(d, &h);
+  rc = get_state(d, V.st, &h);
   if (rc) return rc;
   if (h.bnorm == 0.0) {  // zero right-hand side: the solution is zero
-    MM_HIP(hipMemsetAsync(d->x, 0, d->nrow * sizeof(double), d->stream));
+    k_zero<<<(d->nrow + 255) / 256, 256, 0, d->stream>>>(x, d->nrow);
+    MM_HIP(hipGetLastError());
   }
   while (!h.done) {
     for (int k = 0; k < 32; k++) {
-      k_pcg_spmv<<<d->nslab, d->S, 0, d->stream>>>(d->cols, d->vals, d->nrow, d->nnz, d->S, d->z,
-                                                   d->pA, d->pB, d->q, d->part, d->st);
-      k_reduce<<<1, 256, 0, d->stream>>>(d->part, d->nslab, RED_PQ, d->st);
-      k_pcg_update<<<d->nslab, d->S, 0, d->stream>>>(d->nrow, d->S, d->dinv, d->pA, d->pB, d->q, d->x,
-                                                     d->r, d->z, d->part, d->st);
-      k_reduce<<<1, 256, 0, d->stream>>>(d->part, d->nslab, RED_RR, d->st);
+      k_pcg_spmv<<<d->nslab, d->S, 0, d->stream>>>(d->cols, d->vals, d->nrow, d->nnz, d->S, V.z, V.pA, V.pB,
+                                                   V.q, V.part, V.st);
+      k_reduce<<<1, 256, 0, d->stream>>>(V.part, d->nslab, RED_PQ, V.st);
+      k_pcg_update<<<d->nslab, d->S, 0, d->stream>>>(d->nrow, d->S, d->dinv, V.pA, V.pB, V.q, x, V.r, V.z,
+                                                     V.part, V.st);
+      k_reduce<<<1, 256, 0, d->stream>>>(V.part, d->nslab, RED_RR, V.st);
     }
     MM_HIP(hipGetLastError());
-    rc = get_state(d, &h);
+    rc = get_state(d, V.st, &h);
     if (rc) return rc;
   }
-  MM_HIP(hipMemcpyAsync(x, d->x, d->nrow * sizeof(double), hipMemcpyDeviceToHost, d->stream));
   MM_HIP(hipStreamSynchronize(d->stream));
   const double relres = (h.bnorm == 0.0) ? 0.0 : h.rnorm / h.bnorm;
   if (solver) {
@@ -548,6 +567,92 @@ extern "C" int ell_device_solve_pcg(ell_device *d, ell_solver *solver, const dou
     solver->err = relres;
   }
   return (h.bnorm == 0.0 || h.rnorm <= rtol * h.bnorm) ? 0 : MM_NOT_CONVERGED;
+}
+
+extern "C" int ell_device_solve_pcg(ell_device *d, ell_solver *solver, const double *b, double *x)
+{
+  if (!d || !b || !x) return 1;
+  if (d->nrow != d->ncol) return 1;
+  if (d->nrow == 0) return 0;
+  MM_HIP(hipMemcpyAsync(d->x, x, d->nrow * sizeof(double), hipMemcpyHostToDevice, d->stream));
+  MM_HIP(hipMemcpyAsync(d->b, b, d->nrow * sizeof(double), hipMemcpyHostToDevice, d->stream));
+  const PcgVecs V{d->r, d->z, d->pA, d->pB, d->q, d->part, d->st};
+  const int rc = pcg_run(d, V, solver, d->b, d->x);
+  if (rc != 0 && rc != MM_NOT_CONVERGED) return rc;
+  MM_HIP(hipMemcpyAsync(x, d->x, d->nrow * sizeof(double), hipMemcpyDeviceToHost, d->stream));
+  MM_HIP(hipStreamSynchronize(d->stream));
+  return rc;
+}
+
+// ---- internal: a device ELL filled by kernels (fe.hip) --------------------
+
+int mm::ell_device_alloc(int nrow, int nnz, int S, ell_device **out)
+{
+  if (!out || nrow < 0 || nnz < 1 || S % 64 != 0 || S < 64 || S > 1024) return MM_ERR_ARG;
+  ell_device *d = (ell_device *)calloc(1, sizeof(ell_device));
+  if (!d) return MM_ERR_ALLOC;
+  d->nrow = d->ncol = nrow;
+  d->nnz = nnz;
+  d->S = S;
+  d->nslab = (nrow + S - 1) / S;
+  if (hipGetDevice(&d->device) != hipSuccess) {
+    free(d);
+    return MM_ERR_HIP;
+  }
+  const size_t slots = (size_t)d->nslab * S * nnz;
+  const size_t nv = (size_t)(d->nslab ? d->nslab : 1) * S;
+  int rc = 0;
+#define ALLOC(ptr, bytes)                                                   \
+  if (hipMalloc((void **)&(ptr), (bytes) ? (bytes) : 8) != hipSuccess) { \
+    rc = MM_ERR_HIP;                                                         \
+    goto fail;                                                               \
+  }
+  if (hipStreamCreateWithFlags(&d->stream, hipStreamNonBlocking) != hipSuccess) {
+    free(d);
+    return MM_ERR_HIP;
+  }
+  ALLOC(d->cols, slots * sizeof(int));
+  ALLOC(d->vals, slots * sizeof(double));
+  ALLOC(d->dinv, nv * sizeof(double));
+  ALLOC(d->x, nv * sizeof(double));
+  ALLOC(d->b, nv * sizeof(double));
+  ALLOC(d->r, nv * sizeof(double));
+  ALLOC(d->z, nv * sizeof(double));
+  ALLOC(d->pA, nv * sizeof(double));
+  ALLOC(d->pB, nv * sizeof(double));
+  ALLOC(d->q, nv * sizeof(double));
+  ALLOC(d->part, (size_t)(d->nslab + 1) * 4 * sizeof(double));
+  ALLOC(d->st, sizeof(IterState));
+#undef ALLOC
+  if (hipMemsetAsync(d->cols, 0xff, slots * sizeof(int), d->stream) != hipSuccess ||
+      hipMemsetAsync(d->vals, 0, slots * sizeof(double), d->stream) != hipSuccess ||
+      hipMemsetAsync(d->st, 0, sizeof(IterState), d->stream) != hipSuccess ||
+      hipStreamSynchronize(d->stream) != hipSuccess) {
+    rc = MM_ERR_HIP;
+    goto fail;
+  }
+  *out = d;
+  return 0;
+fail:
+  ell_device_destroy(d);
+  return rc;
+}
+
+void mm::ell_device_arrays(ell_device *d, int **cols, double **vals, double **dinv, int *S, hipStream_t *s)
+{
+  *cols = d->cols;
+  *vals = d->vals;
+  *dinv = d->dinv;
+  *S = d->S;
+  *s = d->stream;
+}
+
+int mm::ell_device_pcg_device(ell_device *d, ell_solver *solver, const double *b, double *x)
+{
+  if (!d || !b || !x) return MM_ERR_ARG;
+  if (d->nrow == 0) return 0;
+  const PcgVecs V{d->r, d->z, d->pA, d->pB, d->q, d->part, d->st};
+  return pcg_run(d, V, solver, b, x);
 }
 
 extern "C" int ell_device_destroy(ell_device *d)

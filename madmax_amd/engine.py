@@ -218,7 +218,7 @@ class Micro:
     """The reference's process-global micro (micro_init + homog_* entry points)."""
 
     def __init__(self, spec, solver=_lib.MICRO_SOLVER_PCG, rtol=1e-10, nl_max_its=2,
-                 nl_min_norm=1e-4, device=0):
+                 nl_min_norm=1e-4, device=0, linear_materials=1, fe2_bc=_lib.MICRO_BC_USTRAIN):
         L = lib()
         c = MicroConfig()
         check(L.micro_config_default(C.byref(c)), "micro_config_default")
@@ -228,6 +228,8 @@ class Micro:
         c.nl_max_its = nl_max_its
         c.nl_min_norm = nl_min_norm
         c.device = device
+        c.linear_materials = int(linear_materials)
+        c.fe2_bc = int(fe2_bc)
         check(L.micro_init(C.byref(c)), "micro_init")
         self.spec = spec
 
@@ -267,7 +269,8 @@ class Micro:
     def stats(self):
         s = MicroStats()
         check(lib().micro_get_stats(C.byref(s)), "micro_get_stats")
-        return dict(pcg_its=s.pcg_its, sweeps=s.sweeps, residual=s.residual, seconds=s.seconds)
+        return dict(pcg_its=s.pcg_its, sweeps=s.sweeps, residual=s.residual, seconds=s.seconds,
+                    assemblies=s.assemblies)
 
     def sizes(self):
         nr, nz, ne = C.c_int(), C.c_int(), C.c_int()
@@ -286,6 +289,15 @@ class Micro:
         et = np.zeros(ne, dtype=np.int32)
         check(lib().micro_get_elem_type(_ip(et)), "micro_get_elem_type")
         return et
+
+    def residual(self, x, strain=None):
+        """assembly_res_ell on the GPU for displacement x: (r, |r|)."""
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        r = np.zeros(len(x))
+        nrm = C.c_double()
+        sp = _dp(np.ascontiguousarray(strain, dtype=np.float64)) if strain is not None else None
+        check(lib().micro_residual(sp, _dp(x), _dp(r), C.byref(nrm)), "micro_residual")
+        return r, nrm.value
 
     def displacement(self):
         nr, _, _ = self.sizes()
@@ -353,3 +365,71 @@ class Ell:
             self.free()
         except Exception:
             pass
+
+
+class Macro:
+    """Macro hex8 mesh assembled on the GPU from gauss-point tangents and
+    stresses (include/macro_api.h)."""
+
+    def __init__(self, n, lengths=(1.0, 1.0, 1.0), coords=None, device=0):
+        nex, ney, nez = (n, n, n) if isinstance(n, (int, np.integer)) else tuple(int(v) for v in n)
+        self.h = C.c_void_p()
+        cp = None
+        if coords is not None:
+            self._coords = np.ascontiguousarray(coords, dtype=np.float64)
+            cp = _dp(self._coords)
+        check(lib().mm_macro_create(nex, ney, nez, float(lengths[0]), float(lengths[1]), float(lengths[2]), cp,
+                                    device, C.byref(self.h)), "mm_macro_create")
+        nr, ne = C.c_int(), C.c_int()
+        lib().mm_macro_sizes(self.h, C.byref(nr), C.byref(ne))
+        self.nrow, self.nelm = nr.value, ne.value
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().mm_macro_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def set_dirichlet(self, dofs):
+        d = np.ascontiguousarray(dofs, dtype=np.int32)
+        check(lib().mm_macro_set_dirichlet(self.h, len(d), _ip(d)), "mm_macro_set_dirichlet")
+
+    def set_u(self, u):
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        check(lib().mm_macro_set_u(self.h, _dp(u)), "mm_macro_set_u")
+
+    def strain(self):
+        out = np.zeros((self.nelm, 8, 6))
+        check(lib().mm_macro_strain(self.h, _dp(out)), "mm_macro_strain")
+        return out
+
+    def assemble_b(self, stress_gp):
+        s = np.ascontiguousarray(stress_gp, dtype=np.float64)
+        b = np.zeros(self.nrow)
+        nrm = C.c_double()
+        check(lib().mm_macro_assemble_b(self.h, _dp(s), _dp(b), C.byref(nrm)), "mm_macro_assemble_b")
+        return b, nrm.value
+
+    def assemble_a(self, c_gp):
+        c = np.ascontiguousarray(c_gp, dtype=np.float64)
+        check(lib().mm_macro_assemble_a(self.h, _dp(c)), "mm_macro_assemble_a")
+
+    def ell(self):
+        cols = np.zeros((self.nrow, 81), dtype=np.int32)
+        vals = np.zeros((self.nrow, 81))
+        check(lib().mm_macro_get_ell(self.h, _ip(cols), _dp(vals)), "mm_macro_get_ell")
+        return cols, vals
+
+    def solve(self, b, rtol=1e-10):
+        b = np.ascontiguousarray(b, dtype=np.float64)
+        du = np.zeros(self.nrow)
+        its = C.c_int()
+        rc = lib().mm_macro_solve(self.h, _dp(b), _dp(du), rtol, C.byref(its))
+        if rc not in (0, _lib.MM_NOT_CONVERGED):
+            check(rc, "mm_macro_solve")
+        return du, its.value, rc

@@ -793,3 +793,238 @@ int or_rve_taylor(or_rve *r, int mode, double *c)
   }
   return inv_small(nvoi, acc, c);
 }
+
+/* ========================================================= periodic BC --
+ * BC_PER_LM (micro/src/homogenize.c:244-260, micro/src/assembly.c:175-220,
+ * 272-354, micro/src/alloc.c:15-16).  The reference writes it for 2D only:
+ * corners get u = E.X, and every non-corner node of the high edge (x1 / y1)
+ * is tied to its partner on the low edge by one Lagrange multiplier per dof,
+ * u+ - u- = E.(X+ - X-).  The 3D extension used here: the 8 corners get
+ * u = E.X; every other node with a coordinate on a high face (ix = nex,
+ * iy = ney or iz = nez) is tied to the node at its wrapped position (each
+ * index taken modulo the element count, which is never on a high face) by
+ * u_s - u_m = E.(X_s - X_m), the multiplier entering the force balance as
+ * +mu on the slave and -mu on the master rows (assembly.c:183-196).
+ * The linear saddle-point system [K G^T; G 0] is solved densely (Gaussian
+ * elimination with partial pivoting, constraint rows scaled by the largest
+ * stiffness diagonal): small RVEs only.  x (length 3 nn, may be NULL)
+ * receives the displacement. */
+int or_rve_homog_per_lm(or_rve *r, const double *strain, double *strain_ave, double *stress_ave,
+                        double *x_out)
+{
+  const or_mesh *m = &r->mesh;
+  if (m->dim != 3) return 1;
+  const int nx = m->nx, ny = m->ny, nz = m->nz, nn = m->nn, nd = r->nd;
+  int *slave_of = (int *)malloc(sizeof(int) * (size_t)nn); /* master node, -1 if none, -2 corner */
+  int nlm = 0;
+  for (int n = 0; n < nn; n++) {
+    const int ix = n % nx, iy = (n / nx) % ny, iz = n / (nx * ny);
+    const int cx = (ix == 0 || ix == nx - 1), cy = (iy == 0 || iy == ny - 1), cz = (iz == 0 || iz == nz - 1);
+    if (cx && cy && cz) {
+      slave_of[n] = -2;
+    } else if (ix == nx - 1 || iy == ny - 1 || iz == nz - 1) {
+      slave_of[n] = (ix % m->nex) + (iy % m->ney) * nx + (iz % m->nez) * nx * ny;
+      nlm++;
+    } else {
+      slave_of[n] = -1;
+    }
+  }
+  const int nu = 3 * nn, N = nu + 3 * nlm;
+  double *A = (double *)calloc((size_t)N * (size_t)(N + 1), sizeof(double));
+  if (!A) { free(slave_of); return 2; }
+#define AA(i, j) A[(size_t)(i) * (N + 1) + (j)]
+  int idx[24];
+  for (int e = 0; e < m->nelm; e++) {
+    or_elem_index(m, e, idx);
+    const double *ke = r->ke[r->elem_type[e]];
+    for (int i = 0; i < nd; i++)
+      for (int j = 0; j < nd; j++) AA(idx[i], idx[j]) += ke[i * nd + j];
+  }
+  double smax = 0.0;
+  for (int i = 0; i < nu; i++) smax = fmax(smax, fabs(AA(i, i)));
+  /* corners: identity rows with u = E.X */
+  for (int n = 0; n < nn; n++) {
+    if (slave_of[n] != -2) continue;
+    double X[3], u[3];
+    or_node_coord(m, n, X);
+    strain_x_coord(3, strain, X, u);
+    for (int d = 0; d < 3; d++) {
+      for (int j = 0; j <= N; j++) AA(3 * n + d, j) = 0.0;
+      AA(3 * n + d, 3 * n + d) = smax;
+      AA(3 * n + d, N) = smax * u[d];
+    }
+  }
+  /* multiplier rows and columns */
+  int l = nu;
+  for (int n = 0; n < nn; n++) {
+    const int mm = slave_of[n];
+    if (mm < 0) continue;
+    double Xs[3], Xm[3], dX[3], u[3];
+    or_node_coord(m, n, Xs);
+    or_node_coord(m, mm, Xm);
+    for (int d = 0; d < 3; d++) dX[d] = Xs[d] - Xm[d];
+    strain_x_coord(3, strain, dX, u);
+    for (int d = 0; d < 3; d++, l++) {
+      AA(l, 3 * n + d) = smax;
+      AA(l, 3 * mm + d) = -smax;
+      AA(l, N) = smax * u[d];
+      if (slave_of[n] >= 0) AA(3 * n + d, l) += smax;
+      if (slave_of[mm] != -2) AA(3 * mm + d, l) -= smax;
+    }
+  }
+#undef AA
+  /* Gaussian elimination with partial pivoting */
+  int ierr = 0;
+  for (int c = 0; c < N && !ierr; c++) {
+    int p = c;
+    for (int i = c + 1; i < N; i++)
+      if (fabs(A[(size_t)i * (N + 1) + c]) > fabs(A[(size_t)p * (N + 1) + c])) p = i;
+    if (A[(size_t)p * (N + 1) + c] == 0.0) { ierr = 3; break; }
+    if (p != c)
+      for (int j = c; j <= N; j++) {
+        double t = A[(size_t)c * (N + 1) + j];
+        A[(size_t)c * (N + 1) + j] = A[(size_t)p * (N + 1) + j];
+        A[(size_t)p * (N + 1) + j] = t;
+      }
+    const double piv = A[(size_t)c * (N + 1) + c];
+    for (int i = c + 1; i < N; i++) {
+      const double f = A[(size_t)i * (N + 1) + c] / piv;
+      if (f == 0.0) continue;
+      for (int j = c; j <= N; j++) A[(size_t)i * (N + 1) + j] -= f * A[(size_t)c * (N + 1) + j];
+    }
+  }
+  double *sol = (double *)malloc(sizeof(double) * (size_t)N);
+  for (int i = N - 1; i >= 0 && !ierr; i--) {
+    double t = A[(size_t)i * (N + 1) + N];
+    for (int j = i + 1; j < N; j++) t -= A[(size_t)i * (N + 1) + j] * sol[j];
+    sol[i] = t / A[(size_t)i * (N + 1) + i];
+  }
+  if (!ierr) {
+    or_rve_averages(r, sol, strain_ave, stress_ave);
+    if (x_out) memcpy(x_out, sol, sizeof(double) * (size_t)nu);
+  }
+  free(sol);
+  free(A);
+  free(slave_of);
+  return ierr;
+}
+
+/* ============================================================= macro hex8 --
+ * Isoparametric hex8 of the macro assembly (macro/src/assembly.c:105-130 for
+ * the stiffness, :20-41 for the residual, :227-244 for dsh), with the
+ * Jacobian, its cofactor inverse and the derivative transform of
+ * src/fem.c:411-471 and the hex8 tables of src/fem.c:309-405 (weights 1).
+ * coor[8][3] in the hex8 node order; B is the 3D Voigt extension of
+ * assembly_get_bmat (xx, yy, zz, xy, yz, xz).  wdet[gp] = |det J|. */
+int or_macro_bmat(const double *coor, double *bmat, double *detj)
+{
+  double xp[8][3], wp[8], sh[8][8], ds[8][3][8];
+  or_hex8_tables(xp, wp, sh, ds);
+  for (int i = 0; i < 6 * 24 * 8; i++) bmat[i] = 0.0;
+  for (int g = 0; g < 8; g++) {
+    double J[3][3], c[3][3], iJ[3][3], det;
+    for (int i = 0; i < 3; i++)
+      for (int j = 0; j < 3; j++) {
+        J[i][j] = 0.0;
+        for (int n = 0; n < 8; n++) J[i][j] += ds[n][i][g] * coor[n * 3 + j];
+      }
+    c[0][0] = +J[1][1] * J[2][2] - J[2][1] * J[1][2];
+    c[0][1] = -J[1][0] * J[2][2] + J[2][0] * J[1][2];
+    c[0][2] = +J[1][0] * J[2][1] - J[2][0] * J[1][1];
+    c[1][0] = -J[0][1] * J[2][2] + J[2][1] * J[0][2];
+    c[1][1] = +J[0][0] * J[2][2] - J[2][0] * J[0][2];
+    c[1][2] = -J[0][0] * J[2][1] + J[2][0] * J[0][1];
+    c[2][0] = +J[0][1] * J[1][2] - J[1][1] * J[0][2];
+    c[2][1] = -J[0][0] * J[1][2] + J[1][0] * J[0][2];
+    c[2][2] = +J[0][0] * J[1][1] - J[1][0] * J[0][1];
+    det = J[0][0] * c[0][0] + J[0][1] * c[0][1] + J[0][2] * c[0][2];
+    for (int i = 0; i < 3; i++)
+      for (int j = 0; j < 3; j++) iJ[i][j] = c[j][i] / det;
+    detj[g] = fabs(det);
+    for (int a = 0; a < 8; a++) {
+      double d[3];
+      for (int i = 0; i < 3; i++) {
+        d[i] = 0.0;
+        for (int j = 0; j < 3; j++) d[i] += iJ[i][j] * ds[a][j][g];
+      }
+#define BM(v, i) bmat[((v) * 24 + (i)) * 8 + g]
+      BM(0, a * 3 + 0) = d[0];
+      BM(1, a * 3 + 1) = d[1];
+      BM(2, a * 3 + 2) = d[2];
+      BM(3, a * 3 + 0) = d[1];
+      BM(3, a * 3 + 1) = d[0];
+      BM(4, a * 3 + 1) = d[2];
+      BM(4, a * 3 + 2) = d[1];
+      BM(5, a * 3 + 0) = d[2];
+      BM(5, a * 3 + 2) = d[0];
+#undef BM
+    }
+  }
+  return 0;
+}
+
+/* k_elem += B[h][i] C[h][k] B[k][j] wp |detJ| per gauss point, loops gp, i,
+ * j, k, h (macro/src/assembly.c:173-181); c_gp[gp][36] row-major. */
+int or_macro_ke(const double *coor, const double *c_gp, double *ke)
+{
+  double bmat[6 * 24 * 8], detj[8];
+  or_macro_bmat(coor, bmat, detj);
+  for (int i = 0; i < 24 * 24; i++) ke[i] = 0.0;
+  for (int g = 0; g < 8; g++)
+    for (int i = 0; i < 24; i++)
+      for (int j = 0; j < 24; j++)
+        for (int k = 0; k < 6; k++)
+          for (int h = 0; h < 6; h++)
+            ke[i * 24 + j] += bmat[(h * 24 + i) * 8 + g] * c_gp[g * 36 + h * 6 + k] * bmat[(k * 24 + j) * 8 + g] *
+                              1.0 * detj[g];
+  return 0;
+}
+
+/* Strain at the gauss points from the element displacement; components
+ * below 1e-6 in magnitude are set to zero (macro/src/assembly.c:307-313). */
+int or_macro_strain(const double *coor, const double *ue, double *strain)
+{
+  double bmat[6 * 24 * 8], detj[8];
+  or_macro_bmat(coor, bmat, detj);
+  for (int g = 0; g < 8; g++)
+    for (int v = 0; v < 6; v++) {
+      double s = 0.0;
+      for (int i = 0; i < 24; i++) s += bmat[(v * 24 + i) * 8 + g] * ue[i];
+      strain[g * 6 + v] = (fabs(s) < 1.0e-6) ? 0.0 : s;
+    }
+  return 0;
+}
+
+/* res_elem[i] += B[j][i] stress_gp[j] wp |detJ| (macro/src/assembly.c:37-40) */
+int or_macro_res(const double *coor, const double *stress, double *res)
+{
+  double bmat[6 * 24 * 8], detj[8];
+  or_macro_bmat(coor, bmat, detj);
+  for (int i = 0; i < 24; i++) res[i] = 0.0;
+  for (int g = 0; g < 8; g++)
+    for (int i = 0; i < 24; i++)
+      for (int j = 0; j < 6; j++) res[i] += bmat[(j * 24 + i) * 8 + g] * stress[g * 6 + j] * 1.0 * detj[g];
+  return 0;
+}
+
+/* MatZeroRowsColumns(A, n, rows, diag) on an ELL (macro/src/assembly.c:143-148):
+ * every listed row keeps its columns with value 0 except diag on the
+ * diagonal, and every listed column is zeroed in every other row. */
+int or_ell_zero_rows_cols(or_ell *m, int n, const int *rows, double diag)
+{
+  char *mark = (char *)calloc((size_t)m->ncol, 1);
+  for (int k = 0; k < n; k++) {
+    if (rows[k] < 0 || rows[k] >= m->nrow || rows[k] >= m->ncol) { free(mark); return 1; }
+    mark[rows[k]] = 1;
+  }
+  for (int i = 0; i < m->nrow; i++)
+    for (int j = 0; j < m->nnz; j++) {
+      const size_t s = (size_t)i * m->nnz + j;
+      const int c = m->cols[s];
+      if (c < 0) break;
+      if (i < m->ncol && mark[i]) m->vals[s] = (c == i) ? diag : 0.0;
+      else if (mark[c]) m->vals[s] = 0.0;
+    }
+  free(mark);
+  return 0;
+}

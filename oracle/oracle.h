@@ -127,6 +127,24 @@ int or_rve_c_tangent_unit(or_rve *r, double amp, double rtol, double *c, int *it
  * (micro/src/homogenize.c:96-163); LU inverse restated with Gauss-Jordan. */
 int or_rve_taylor(or_rve *r, int mode, double *c);
 
+/* BC_PER_LM homogenization (micro/src/assembly.c:175-220, 272-354, 2D in the
+ * reference; 3D extension in oracle.c), solved densely: small RVEs only. */
+int or_rve_homog_per_lm(or_rve *r, const double *strain, double *strain_ave, double *stress_ave,
+                        double *x);
+
+/* ------------------------------------------------------------ macro hex8 --
+ * Isoparametric hex8 (macro/src/assembly.c, src/fem.c:411-471): B and
+ * |det J| at the 8 gauss points from nodal coordinates coor[8][3],
+ * k_elem with a tangent per gauss point, the gauss-point strains (with the
+ * 1e-6 flush of assembly.c:312) and the element residual from gauss-point
+ * stresses. */
+int or_macro_bmat(const double *coor, double *bmat, double *detj);
+int or_macro_ke(const double *coor, const double *c_gp, double *ke);
+int or_macro_strain(const double *coor, const double *ue, double *strain);
+int or_macro_res(const double *coor, const double *stress, double *res);
+/* MatZeroRowsColumns semantics on an ELL (macro/src/assembly.c:143-148). */
+int or_ell_zero_rows_cols(or_ell *m, int n, const int *rows, double diag);
+
 #ifdef __cplusplus
 }
 #endif

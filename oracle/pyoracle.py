@@ -82,6 +82,12 @@ def lib():
         L.or_elem_centroid.argtypes = [C.POINTER(OrMesh), C.c_int, dp]
         L.or_boundary_nodes.argtypes = [C.POINTER(OrMesh), ip]
         L.or_phase_map.argtypes = [C.POINTER(OrMesh), C.c_int, dp, dp, dp, ip]
+        L.or_rve_homog_per_lm.argtypes = [C.c_void_p, dp, dp, dp, dp]
+        L.or_macro_bmat.argtypes = [dp, dp, dp]
+        L.or_macro_ke.argtypes = [dp, dp, dp]
+        L.or_macro_strain.argtypes = [dp, dp, dp]
+        L.or_macro_res.argtypes = [dp, dp, dp]
+        L.or_ell_zero_rows_cols.argtypes = [C.POINTER(OrEll), C.c_int, ip, C.c_double]
         _LIB = L
     return _LIB
 
@@ -221,6 +227,20 @@ class Rve:
             raise RuntimeError("oracle PCG did not converge")
         return c, its.value
 
+    def homog_per_lm(self, strain):
+        """BC_PER_LM (periodic, Lagrange multipliers) solved densely; small RVEs."""
+        s = np.ascontiguousarray(strain, dtype=np.float64)
+        se = np.zeros(self.nvoi); ss = np.zeros(self.nvoi); x = np.zeros(self.nrow)
+        if lib().or_rve_homog_per_lm(self.h, _p(s), _p(se), _p(ss), _p(x)):
+            raise RuntimeError("singular periodic system")
+        return dict(strain_ave=se, stress_ave=ss, x=x)
+
+    def c_tangent_per_lm(self, amp=1e-3):
+        c = np.zeros((6, 6))
+        for i in range(6):
+            c[:, i] = self.homog_per_lm(amp * np.eye(6)[i])["stress_ave"] / amp
+        return c
+
     def taylor(self, mode):
         c = np.zeros((self.nvoi, self.nvoi))
         if lib().or_rve_taylor(self.h, mode, _p(c)):
@@ -235,3 +255,41 @@ def pcg(ell_ptr, b, x0=None, rtol=1e-10, max_its=100000):
     its = C.c_int(); rr = C.c_double()
     ierr = lib().or_pcg(ell_ptr, _p(b), _p(x), rtol, max_its, C.byref(its), C.byref(rr))
     return x, its.value, rr.value, ierr
+
+
+# ------------------------------------------------------------- macro hex8
+
+def macro_bmat(coor):
+    coor = np.ascontiguousarray(coor, dtype=np.float64).reshape(8, 3)
+    b = np.zeros((6, 24, 8)); dj = np.zeros(8)
+    lib().or_macro_bmat(_p(coor), _p(b), _p(dj))
+    return b, dj
+
+
+def macro_ke(coor, c_gp):
+    coor = np.ascontiguousarray(coor, dtype=np.float64).reshape(8, 3)
+    c = np.ascontiguousarray(np.broadcast_to(c_gp, (8, 6, 6)), dtype=np.float64)
+    ke = np.zeros((24, 24))
+    lib().or_macro_ke(_p(coor), _p(c), _p(ke))
+    return ke
+
+
+def macro_strain(coor, ue):
+    coor = np.ascontiguousarray(coor, dtype=np.float64).reshape(8, 3)
+    ue = np.ascontiguousarray(ue, dtype=np.float64)
+    st = np.zeros((8, 6))
+    lib().or_macro_strain(_p(coor), _p(ue), _p(st))
+    return st
+
+
+def macro_res(coor, stress_gp):
+    coor = np.ascontiguousarray(coor, dtype=np.float64).reshape(8, 3)
+    s = np.ascontiguousarray(np.broadcast_to(stress_gp, (8, 6)), dtype=np.float64)
+    r = np.zeros(24)
+    lib().or_macro_res(_p(coor), _p(s), _p(r))
+    return r
+
+
+def ell_zero_rows_cols(ell_ptr, rows, diag=1.0):
+    rows = np.ascontiguousarray(rows, dtype=np.int32)
+    return lib().or_ell_zero_rows_cols(ell_ptr, len(rows), _p(rows, ip), diag)

@@ -19,7 +19,7 @@ GOLD = os.path.join(ROOT, "tests", "golden")
 
 def declared_functions():
     names = []
-    for h in ("ell.h", "micro_api.h"):
+    for h in ("ell.h", "micro_api.h", "macro_api.h"):
         txt = open(os.path.join(ROOT, "include", h)).read()
         txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
         for m in re.finditer(r"^\s*(?:int|double)\s+\**\s*(\w+)\s*\(", txt, flags=re.M):

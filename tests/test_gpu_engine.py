@@ -403,3 +403,31 @@ def test_warm_start_superposes_kept_solutions():
         assert normwise_rel(r["stress"][0], st_ref) <= TOL
     print("increment warm its", res[1]["pcg_its"], "cold its", res[0]["pcg_its"], flush=True)
     assert res[1]["pcg_its"] < res[0]["pcg_its"]
+
+
+def test_dictionary_gather_beyond_2_30_bytes():
+    """The dictionary SpMV reads z through a buffer descriptor with 32-bit
+    byte offsets.  A 190^3 RVE at 7 FP64 load cases has a 1.17 GB vector run
+    (above 2^30 bytes): a few PCG iterations with dictionary storage give the
+    same bits as full storage, whose gather uses 64-bit addresses."""
+    spec = rve_spec(190, [(0.5, 0.5, 0.3)])
+    loads = [1e-3 * np.eye(6)[i % 6] for i in range(7)]
+    nodes = 191 ** 3
+    run = -(-nodes // 256) * 7 * 3 * 256 * 8
+    assert run > (1 << 30)
+    out = {}
+    for storage in (MM_ELL_DICT, MM_ELL_FULL):
+        budget = int(3 * nodes * (81 * 8 + 40 + 7 * 40 + 8) * 1.15) + 2 * GB
+        print("190^3 FP64 7 loads, storage", storage, flush=True)
+        with Engine(nrhs=7, precision=MM_PREC_FP64, hbm_budget=budget, max_rves=2, slab_nodes=256,
+                    max_inner_its=3, max_sweeps=1, inner_rtol=1e-14, rtol=1e-14, ell_storage=storage) as eng:
+            assert eng.submit(1, spec, loads, MM_TANGENT_NONE, warm_key=1) == 0
+            eng.drain()
+            r = eng.poll()[0]
+            x = eng.download_x(1, 6)
+        out[storage] = (r, x)
+    (rd, xd), (rf, xf) = out[MM_ELL_DICT], out[MM_ELL_FULL]
+    assert rd["pcg_its"] == rf["pcg_its"] == 3
+    assert np.array_equal(xd, xf)
+    assert np.array_equal(rd["stress"], rf["stress"])
+    assert np.any(xd != 0.0)

@@ -108,13 +108,15 @@ def test_c4_512_uneven_batch():
 
 
 def _c5_engine():
-    return Engine(nrhs=7, precision=MM_PREC_MIXED, hbm_budget=48 * GB, max_rves=4096, slab_nodes=256,
+    return Engine(nrhs=7, precision=MM_PREC_MIXED, hbm_budget=48 * GB, max_rves=8192, slab_nodes=256,
                   inner_rtol=1e-5, rtol=1e-9, steps_per_launch=8)
 
 
 def test_c5_full_stream():
     g = np.load(os.path.join(GOLD, "wl_c5.npz"))
-    rate = 2000.0
+    # below the engine's capacity for this mix (about 1,500 requests/s), so the
+    # backlog stays short and the 64 kept RVEs are not evicted to make room
+    rate = 1000.0
     stream = configs.c5_stream(rate=rate)
     assert len(stream) == 32768
     sample = [int(i) for i in g["index"]]

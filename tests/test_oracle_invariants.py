@@ -227,3 +227,26 @@ def test_hill_mandel_energy():
         out = r.homog(eps, rtol=1e-13)
         u = out["x"]
         assert np.isclose(u @ K @ u / m.vol, out["stress_ave"] @ eps, rtol=1e-9)
+
+
+def test_taylor_bounds_homogeneous_and_two_phase_closed_form():
+    """or_rve_taylor (micro/src/homogenize.c:96-163): on a homogeneous cell
+    both mixture rules return C itself; on a two-phase cell they equal the
+    closed forms vf-weighted sum of C (Voigt) and the inverse of the
+    vf-weighted sum of C^-1 (Reuss), with vf counted from the phase map, so a
+    dropped or swapped volume fraction fails."""
+    import numpy as np
+    from madmax_amd import rve_spec, spec_fibers
+    from oracle import pyoracle as po
+    from tests.helpers import normwise_rel, oracle_rve
+    r, _ = oracle_rve(rve_spec(4, [], young=(3e7, 3e7), poisson=(0.2, 0.2)))
+    c = po.material_c(3, 3e7, 0.2)
+    assert normwise_rel(r.taylor(0), c) <= 1e-15
+    assert normwise_rel(r.taylor(1), c) <= 1e-14
+    spec = rve_spec(6, [(0.5, 0.5, 0.3)])
+    r, et = oracle_rve(spec)
+    vf = float(np.mean(et))
+    assert 0.1 < vf < 0.5
+    c0, c1 = po.material_c(3, 1e7, 0.25), po.material_c(3, 1e8, 0.3)
+    assert normwise_rel(r.taylor(0), (1 - vf) * c0 + vf * c1) <= 1e-14
+    assert normwise_rel(r.taylor(1), np.linalg.inv((1 - vf) * np.linalg.inv(c0) + vf * np.linalg.inv(c1))) <= 1e-12
