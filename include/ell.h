@@ -106,6 +106,19 @@ int ell_device_solve_pcg(ell_device *d, ell_solver *solver, const double *b, dou
 int ell_device_solve_jacobi(ell_device *d, ell_solver *solver, const double *b, double *x);
 int ell_device_destroy(ell_device *d);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): the Jacobi-PCG of ell_device_solve_pcg (same kernels,
+ * same bits) on a mirror created beforehand, with b and x (initial guess in,
+ * solution out) in device memory the caller owns.  The PCG work vectors,
+ * partial sums and iteration state live in one library-wide device array
+ * sized for ELL_NOALLOC_MAX_ROWS rows (about 40.5 MiB per GPU, held until the
+ * process ends); the call allocates nothing.  Returns 1 for a matrix with more
+ * rows, 0 when converged, 5 at max_its.  Every call on one GPU uses the same
+ * array: calls that overlap on one GPU race. */
+#define ELL_NOALLOC_MAX_ROWS (1 << 20)
+int ell_device_solve_pcg_noalloc(ell_device *d, ell_solver *solver, const double *b, double *x);
+
 #ifdef __cplusplus
 }
 #endif

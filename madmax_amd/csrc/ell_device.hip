@@ -655,6 +655,67 @@ int mm::ell_device_pcg_device(ell_device *d, ell_solver *solver, const double *b
   return pcg_run(d, V, solver, b, x);
 }
 
+// ---- the same solve without any allocation ----------------------------------
+// The PCG work vectors, partial sums and iteration state of
+// ell_device_solve_pcg_noalloc live in this file-scope array: five vectors of
+// ELL_NOALLOC_MAX_ROWS doubles, the partials of up to ELL_NOALLOC_MAX_ROWS / 64
+// slabs and one IterState (about 40.5 MiB, held on every GPU the library runs
+// on until the process ends).  Host code never uses the array's name as an
+// address: the per-device address comes from hipGetSymbolAddress.
+namespace {
+constexpr size_t kNaRows = ELL_NOALLOC_MAX_ROWS;
+constexpr size_t kNaPart = (kNaRows / 64 + 1) * 4;
+constexpr size_t kNaState = (sizeof(IterState) + 7) / 8;
+constexpr size_t kNaDoubles = 5 * kNaRows + kNaPart + kNaState;
+constexpr int kNaMaxDevices = 64;
+}  // namespace
+
+static __device__ __align__(256) double scratch[kNaDoubles];
+
+static std::mutex g_na_mu;
+static double *g_na_ptr[kNaMaxDevices];
+
+// Address of `scratch` on `device`, looked up once per device with that
+// device current.
+static int noalloc_scratch(int device, double **out)
+{
+  if (device < 0 || device >= kNaMaxDevices) return MM_ERR_ARG;
+  std::lock_guard<std::mutex> lk(g_na_mu);
+  if (!g_na_ptr[device]) {
+    int cur = -1;
+    MM_HIP(hipGetDevice(&cur));
+    if (cur != device) MM_HIP(hipSetDevice(device));
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (cur != device) MM_HIP(hipSetDevice(cur));
+    if (e != hipSuccess || !p) {
+      fprintf(stderr, "madmax_amd: hipGetSymbolAddress(scratch) failed: %s\n", hipGetErrorName(e));
+      return MM_ERR_HIP;
+    }
+    g_na_ptr[device] = (double *)p;
+  }
+  *out = g_na_ptr[device];
+  return 0;
+}
+
+extern "C" int ell_device_solve_pcg_noalloc(ell_device *d, ell_solver *solver, const double *b, double *x)
+{
+  if (!d || !b || !x) return 1;
+  if (d->nrow != d->ncol) return 1;
+  if ((size_t)d->nrow > kNaRows) {
+    fprintf(stderr, "madmax_amd: ell_device_solve_pcg_noalloc: %d rows exceed the %zu-row scratch\n", d->nrow,
+            kNaRows);
+    return 1;
+  }
+  if (d->nrow == 0) return 0;
+  double *sp = nullptr;
+  const int rc = noalloc_scratch(d->device, &sp);
+  if (rc) return rc;
+  const PcgVecs V{sp, sp + kNaRows, sp + 2 * kNaRows, sp + 3 * kNaRows, sp + 4 * kNaRows, sp + 5 * kNaRows,
+                  reinterpret_cast<IterState *>(sp + 5 * kNaRows + kNaPart)};
+  return pcg_run(d, V, solver, b, x);
+}
+
 extern "C" int ell_device_destroy(ell_device *d)
 {
   if (!d) return 1;
