@@ -57,7 +57,7 @@ sys.path.insert(0, ROOT)
 import numpy as np  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "pmc_k_inner_time_128x30cube.md")
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "pmc_k_step_a_in_solve_128x30cube.md")
 GOLD = os.path.join(ROOT, "tests", "golden")
 METRIC = "RVE tangents/sec at 30³ hex8 (rel err≤1e-8); p99 latency; SpMV HBM GB/s"
 
@@ -83,9 +83,9 @@ def run_batch(eng, specs, mm, warm=False):
 
 
 def measured_traffic(batch, n):
-    """HBM bytes per launch of the step kernel from the committed counter runs
-    (FETCH_SIZE + WRITE_SIZE, calibrated on a copy kernel), if they were
-    taken on this workload."""
+    """HBM bytes per launch of kernel A in the solve from the committed
+    counter runs (FETCH_SIZE + WRITE_SIZE, calibrated on a copy kernel), if
+    they were taken on this workload."""
     if batch != 128 or n != 30 or not os.path.exists(TRAFFIC_FILE):
         return None
     for line in open(TRAFFIC_FILE):
@@ -623,8 +623,9 @@ def main():
                    "rtol": args.rtol, "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "traffic_source": ("constant from profiles/pmc_k_inner_time_128x30cube.md (rocprofv3 "
-                                        "FETCH_SIZE/WRITE_SIZE of the replica, copy-calibrated)"
+                     "traffic_source": ("constant from profiles/pmc_k_step_a_in_solve_128x30cube.md "
+                                        "(rocprofv3 FETCH_SIZE / WRITE_SIZE of kernel A's launches in a solve, "
+                                        "calibrated on the copy kernel); not measured in this run"
                                         if traffic else None),
                      "kernel": "k_step_a<float,6,true> (kernel A: SpMV + p/q update, dictionary ELL) inside a "
                                "solve of %d RVEs: median of %d launches" % (B, len(a_ms)),
