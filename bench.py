@@ -57,7 +57,7 @@ sys.path.insert(0, ROOT)
 import numpy as np  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "pmc_k_step_a_in_solve_128x30cube.md")
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "pmc_k_step_a_march_in_solve_128x30cube.md")
 GOLD = os.path.join(ROOT, "tests", "golden")
 METRIC = "RVE tangents/sec at 30³ hex8 (rel err≤1e-8); p99 latency; SpMV HBM GB/s"
 
@@ -256,9 +256,10 @@ def stream_stats(got, seconds):
                 mean_pcg_its=float(its.mean()), all_ok=all(r["status"] == 0 for r in rs))
 
 
-def c5_engine(mm, count):
+def c5_engine(mm, count, march_planes=-1):
+    kw = {"march_planes": march_planes} if march_planes >= 0 else {}
     return mm.Engine(nrhs=7, precision=mm.MM_PREC_MIXED, hbm_budget=48 << 30, max_rves=min(count + 64, 4096),
-                     slab_nodes=256, inner_rtol=1e-5, rtol=1e-9, steps_per_launch=8)
+                     slab_nodes=256, inner_rtol=1e-5, rtol=1e-9, steps_per_launch=8, **kw)
 
 
 def workload_c5(args, mm, configs):
@@ -269,7 +270,7 @@ def workload_c5(args, mm, configs):
     rates = [float(r) for r in args.rates.split(",")] if args.rates else [1000.0, 2000.0, 3000.0, 4000.0, 6000.0]
     for rate in rates:
         stream = configs.c5_stream(count=count, rate=rate)
-        with c5_engine(mm, count) as se:
+        with c5_engine(mm, count, args.march_planes) as se:
             se.start()
             got, ts = serve_stream(mm, se, stream, 1, False)
             se.stop()
@@ -291,7 +292,7 @@ def workload_c5(args, mm, configs):
     sample = [int(i) for i in g["index"]] if g is not None else list(range(37, len(stream), 512))
     for i in sample:
         stream[i]["warm_key"] = stream[i]["key"]
-    with c5_engine(mm, len(stream)) as se:
+    with c5_engine(mm, len(stream), args.march_planes) as se:
         se.start()
         got, ts = serve_stream(mm, se, stream, 1, False)
         cold = stream_stats(got, ts)
@@ -370,6 +371,8 @@ def main():
     ap.add_argument("--serve", type=int, default=1024, help="requests in the serving sample")
     ap.add_argument("--serve-rate", type=float, default=1000.0, help="arrival rate of the serving sample")
     ap.add_argument("--no-full-ell", action="store_true", help="skip the full-ELL SpMV figure")
+    ap.add_argument("--march-planes", type=int, default=-1,
+                    help="kernel A's plane-march run length (0: slab by slab; -1: engine default)")
     ap.add_argument("--kernels-only", action="store_true")
     ap.add_argument("--workload", default="c2", choices=("c2", "c3", "c4", "c5"))
     ap.add_argument("--rate", type=float, default=0.0, help="c5 full-stream arrival rate (requests/s)")
@@ -410,6 +413,8 @@ def main():
         cfg = dict(nrhs=6, precision=mm.MM_PREC_MIXED, hbm_budget=budget, max_rves=B + 8, slab_nodes=args.slab,
                    inner_rtol=args.inner_rtol, rtol=args.rtol, steps_per_launch=args.spl, use_graph=True,
                    ell_storage=mm.MM_ELL_DICT, solver_streams=args.streams)
+        if args.march_planes >= 0:
+            cfg["march_planes"] = args.march_planes
         cfg.update(kw)
         return mm.Engine(**cfg)
 
@@ -595,7 +600,7 @@ def main():
     serving = None
     if args.serve > 0:
         stream = configs.c5_stream(count=args.serve, rate=args.serve_rate)
-        with c5_engine(mm, args.serve) as se:
+        with c5_engine(mm, args.serve, args.march_planes) as se:
             se.start()
             got, ts = serve_stream(mm, se, stream, 1, False, limit_s=300)
             se.stop()
@@ -623,12 +628,13 @@ def main():
                    "rtol": args.rtol, "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "traffic_source": ("constant from profiles/pmc_k_step_a_in_solve_128x30cube.md "
+                     "traffic_source": ("constant from profiles/pmc_k_step_a_march_in_solve_128x30cube.md "
                                         "(rocprofv3 FETCH_SIZE / WRITE_SIZE of kernel A's launches in a solve, "
                                         "calibrated on the copy kernel); not measured in this run"
                                         if traffic else None),
-                     "kernel": "k_step_a<float,6,true> (kernel A: SpMV + p/q update, dictionary ELL) inside a "
-                               "solve of %d RVEs: median of %d launches" % (B, len(a_ms)),
+                     "kernel": "k_step_a_march_dict<float,6> (kernel A: SpMV + p/q update, dictionary ELL, "
+                               "z-plane march through LDS) inside a solve of %d RVEs: median of %d launches"
+                               % (B, len(a_ms)),
                      "avg_ms": in_solve_ms, "bytes_per_launch": step_bytes,
                      "replica_ms": rep_ms, "replica_frac": step_bytes / (rep_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
                      "ceiling_gbs": ceiling},

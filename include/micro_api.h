@@ -202,6 +202,14 @@ typedef struct {
   int profile_step_kernel; /* 1: no graph, one stream, and every launch of the
                             PCG step kernel (kernel A: SpMV + p/q update)
                             timed with HIP events (mm_engine_step_kernel_times) */
+  int march_planes;      /* kernel A: 0 works slab by slab, gathering every
+                            neighbour from memory; n > 0 (default 8) marches
+                            in-plane chunks of slab_nodes nodes through runs
+                            of n z-planes with the neighbour planes staged in
+                            LDS (MM_PREC_MIXED with nrhs >= 2 only; RVEs
+                            whose x-extent exceeds slab_nodes/2 - 1 elements
+                            stay slab by slab).  Same SpMV bits; the p.q
+                            partial sums are grouped differently. */
 } mm_engine_config;
 
 typedef struct {

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <atomic>
 #include <chrono>
 #include <climits>
@@ -206,14 +207,19 @@ struct mm_engine {
   // solver launches run over the slabs of the RVEs being solved, one list
   // per stream group g at act_list + g * (nslab_cap + 1): [0] is the count,
   // [1..] the slabs in ascending order (device), staged in st_act (pinned);
-  // rebuilt when an RVE is admitted or retired
+  // rebuilt when an RVE is admitted or retired.  With the plane march
+  // (march_planes > 0) kernel A runs over the tiles of the RVEs that march,
+  // listed by their slabs at act_list + (2 + g) * (nslab_cap + 1), with
+  // a_lds[g] bytes of dynamic LDS; a_slab[g]: the group also holds RVEs
+  // that go slab by slab.
   int *act_list = nullptr, *st_act = nullptr;
   bool act_dirty = true;
-  long long act_n[2] = {0, 0};
-  // one captured graph per pair of launch grids (powers of two >= 64 covering
-  // each group's slabs, 0 for an empty group); the kernels read the active
-  // count from their list
-  std::map<std::pair<int, int>, std::pair<hipGraph_t, hipGraphExec_t>> graphs;
+  long long act_n[2] = {0, 0}, tile_n[2] = {0, 0};
+  int a_lds[2] = {0, 0}, a_slab[2] = {0, 0};
+  // one captured graph per set of launch shapes (grids: powers of two >= 64
+  // covering each group's slabs and tiles, 0 for an empty group; kernel A's
+  // LDS); the kernels read the active count from their list
+  std::map<std::array<int, 8>, std::pair<hipGraph_t, hipGraphExec_t>> graphs;
   // stats / threading
   std::mutex step_mu;
   long long completed = 0, nsteps = 0;
@@ -251,6 +257,10 @@ struct mm_engine {
     P.elem_mat = elem_mat;
     P.npat = npat;
     P.act_list = act_list;  // group 0; run_steps points group 1 at its list
+    P.act_tile = act_list + 2 * ((size_t)nslab_cap + 1);
+    P.a_grid = 0;
+    P.a_lds = 0;
+    P.a_slab = 0;
     P.dict = dict;
     P.dict64 = dict64;
     P.SN = SN;
@@ -332,6 +342,7 @@ extern "C" int mm_engine_config_default(mm_engine_config *c)
   c->initial_guess = MM_GUESS_AFFINE;
   c->solver_streams = 2;
   c->profile_step_kernel = 0;
+  c->march_planes = 8;
   return 0;
 }
 
@@ -348,6 +359,7 @@ extern "C" int mm_engine_create(const mm_engine_config *cfg, mm_engine **out)
   if (cfg->ell_storage != MM_ELL_FULL && cfg->ell_storage != MM_ELL_DICT) return MM_ERR_ARG;
   if (cfg->initial_guess != MM_GUESS_ZERO && cfg->initial_guess != MM_GUESS_AFFINE) return MM_ERR_ARG;
   if (cfg->solver_streams != 1 && cfg->solver_streams != 2) return MM_ERR_ARG;
+  if (cfg->march_planes < 0 || cfg->march_planes > 4096) return MM_ERR_ARG;
   if (!(cfg->rtol >= 0) || !(cfg->inner_rtol > 0) || cfg->max_inner_its < 1 || cfg->max_sweeps < 1)
     return MM_ERR_ARG;
   int ndev = 0;
@@ -433,7 +445,7 @@ extern "C" int mm_engine_create(const mm_engine_config *cfg, mm_engine **out)
     else EALLOC(e->dict, (size_t)e->rve_cap * kDictPat * kDictEntry * 4);
   }
   EALLOC(e->npat, (size_t)e->nslab_cap * e->SN);
-  EALLOC(e->act_list, 2 * ((size_t)e->nslab_cap + 1) * 4);
+  EALLOC(e->act_list, 4 * ((size_t)e->nslab_cap + 1) * 4);
   EALLOC(e->dinv, nrows * e->tsz);
   EALLOC(e->x64, nrows * e->K * 8);
   EALLOC(e->d, nrows * e->K * e->tsz);
@@ -455,7 +467,7 @@ extern "C" int mm_engine_create(const mm_engine_config *cfg, mm_engine **out)
     goto fail;
   }
   memset(e->out_host, 0, (size_t)e->rve_cap * sizeof(RveOut));
-  if (hipHostMalloc((void **)&e->st_act, 2 * ((size_t)e->nslab_cap + 1) * 4, hipHostMallocDefault) !=
+  if (hipHostMalloc((void **)&e->st_act, 4 * ((size_t)e->nslab_cap + 1) * 4, hipHostMallocDefault) !=
       hipSuccess) {
     rc = MM_ERR_HIP;
     goto fail;
@@ -661,6 +673,65 @@ static void warm_coefficients(const double (*old)[MM_NVOI], int nold, const doub
   }
 }
 
+// Plane-march tiling of an RVE of nx x ny x nz nodes in nslab slabs (kernel
+// A's tiles, RveDev::mcpp ..): in-plane chunks of at most SN nodes times
+// runs of at most march_planes z-planes (even splits) times groups of load
+// cases, the run doubled until the tiles fit the RVE's slabs.  mcpp = 0 (slab by slab) when the march is
+// off, in FP64 engines, with one load case (27 gathers per node cost less
+// than a tile's fixed work), or when the window (SN + 2 nx + 2 nodes) is
+// longer than two positions per lane or the march's LDS exceeds its bound.  Fixed by
+// the RVE and the engine configuration alone, so an RVE's bits do not depend
+// on the RVEs solved beside it.
+struct MarchTiling {
+  int mcpp = 0, mL = 0, mnzc = 0, mnt = 0, mncg = 0;
+  long long tiles() const { return (long long)mnt * mncg; }
+};
+static MarchTiling march_tiling(const mm_engine *e, const mm_rve_spec &s, long long nslab)
+{
+  MarchTiling m;
+  const int nx = s.nex + 1, ny = s.ney + 1, nz = s.nez + 1, SN = e->SN;
+  if (e->cfg.march_planes <= 0 || 2 * nx + 2 > SN) return m;
+  if (e->fp64 || e->K < 2 || march_lds_bytes(SN, nx, e->K, (int)e->tsz, e->dict_mode) > kMarchMaxLds) return m;
+  const long long cpp = ((long long)nx * ny + SN - 1) / SN;
+  long long L = e->cfg.march_planes;
+  // an RVE of 2048+ slabs can fill the GPU alone (a tile per 1024 slabs of
+  // it keeps every CU busy): its runs shrink to the power of two nearest
+  // below nslab / 1024 planes (at least 2)
+  if (nslab >= 2048) {
+    long long cap = 2;
+    while (cap * 2 <= nslab / 1024) cap *= 2;
+    L = std::min(L, cap);
+  }
+  const long long ncg = (e->K + march_cases((int)e->tsz) - 1) / march_cases((int)e->tsz);
+  if (cpp * ncg > nslab) return m;  // too few slabs for one tile per chunk and case group
+  while (cpp * ((nz + L - 1) / L) * ncg > nslab) L *= 2;
+  m.mcpp = (int)cpp;
+  m.mL = (int)L;
+  m.mnzc = (int)((nz + L - 1) / L);
+  m.mnt = m.mcpp * m.mnzc;
+  m.mncg = (int)ncg;
+  return m;
+}
+
+// Dynamic LDS of a plane-march launch: the need rounded up to the largest
+// size that still fits as many workgroups per CU (160 KB: four up to 40 KB,
+// three up to 53 KB, less static LDS of at most 3 KB), so that RVEs of
+// different widths share launch shapes and graphs.
+static int march_lds_class(int need)
+{
+  const int cls[3] = {40960 - 3072, 54272 - 3072, kMarchMaxLds};
+  for (int c : cls)
+    if (need <= c) return c;
+  return kMarchMaxLds;
+}
+
+// Dynamic LDS of the plane march for an RVE (0: the RVE goes slab by slab).
+static int march_lds_need(const mm_engine *e, const HostSlot &h)
+{
+  const MarchTiling m = march_tiling(e, h.req.rve, h.nslab);
+  return m.mcpp > 0 ? march_lds_bytes(e->SN, h.req.rve.nex + 1, e->K, (int)e->tsz, e->dict_mode) : 0;
+}
+
 static void fill_state(const mm_engine *e, const HostSlot &h, RveDev *st, int warm)
 {
   memset(st, 0, sizeof(*st));
@@ -695,6 +766,12 @@ static void fill_state(const mm_engine *e, const HostSlot &h, RveDev *st, int wa
   st->nload = h.req.nload;
   st->warm = warm;
   st->affine = (e->cfg.initial_guess == MM_GUESS_AFFINE) ? 1 : 0;
+  const MarchTiling mt = march_tiling(e, s, h.nslab);
+  st->mcpp = mt.mcpp;
+  st->mL = mt.mL;
+  st->mnzc = mt.mnzc;
+  st->mnt = mt.mnt;
+  st->mncg = mt.mncg;
   for (int k = 0; k < h.req.nload; k++)
     for (int i = 0; i < MM_NVOI; i++) st->load[k][i] = h.req.strain[k][i];
   if (warm)
@@ -858,28 +935,46 @@ static int solver_grid(const mm_engine *e, long long hw)
 static int update_active(mm_engine *e)
 {
   if (!e->act_dirty) return 0;
-  std::vector<std::pair<long long, long long>> runs;
+  std::vector<std::pair<long long, int>> runs;  // (first slab, slot)
   long long total = 0;
   for (int s = 0; s < e->rve_cap; s++)
     if (e->slots[s].state == SL_ACTIVE) {
-      runs.push_back({e->slots[s].first_slab, e->slots[s].nslab});
+      runs.push_back({e->slots[s].first_slab, s});
       total += e->slots[s].nslab;
     }
   std::sort(runs.begin(), runs.end());
   const size_t stride = (size_t)e->nslab_cap + 1;
   const int ngroup = e->stream2 ? 2 : 1;
-  long long n[2] = {0, 0};
+  const bool march = e->cfg.march_planes > 0 && !e->fp64;
+  long long n[2] = {0, 0}, nt[2] = {0, 0};
+  int lds[2] = {0, 0}, slab_mode[2] = {0, 0};
   int g = 0;
   for (auto &r : runs) {
+    const HostSlot &h = e->slots[r.second];
     if (g == 0 && ngroup == 2 && n[0] > 0 && 2 * n[0] >= total) g = 1;
-    for (long long b = 0; b < r.second; b++) e->st_act[g * stride + 1 + n[g]++] = (int)(r.first + b);
+    for (long long b = 0; b < h.nslab; b++) e->st_act[g * stride + 1 + n[g]++] = (int)(r.first + b);
+    const MarchTiling m = march_tiling(e, h.req.rve, h.nslab);
+    if (march && m.mcpp > 0) {
+      for (long long t = 0; t < m.tiles(); t++) e->st_act[(2 + g) * stride + 1 + nt[g]++] = (int)(r.first + t);
+      lds[g] = std::max(lds[g], march_lds_need(e, h));
+    } else {
+      slab_mode[g] = 1;
+    }
   }
   for (int q = 0; q < 2; q++) {
     e->st_act[q * stride] = (int)n[q];
-    if (q < ngroup)
+    e->st_act[(2 + q) * stride] = (int)nt[q];
+    if (q < ngroup) {
       MM_HIP(hipMemcpyAsync(e->act_list + q * stride, e->st_act + q * stride, (size_t)(n[q] + 1) * 4,
                             hipMemcpyHostToDevice, e->stream));
+      if (march)
+        MM_HIP(hipMemcpyAsync(e->act_list + (2 + q) * stride, e->st_act + (2 + q) * stride,
+                              (size_t)(nt[q] + 1) * 4, hipMemcpyHostToDevice, e->stream));
+    }
     e->act_n[q] = n[q];
+    e->tile_n[q] = nt[q];
+    e->a_lds[q] = (march && nt[q] > 0) ? march_lds_class(lds[q]) : 0;
+    e->a_slab[q] = slab_mode[q];
   }
   e->act_dirty = false;
   return 0;
@@ -903,6 +998,13 @@ static int queue_steps(mm_engine *e, int grid0, int grid1)
 {
   EngPtrs P0 = e->ptrs(), P1 = P0;
   P1.act_list = e->act_list + ((size_t)e->nslab_cap + 1);
+  P1.act_tile = e->act_list + 3 * ((size_t)e->nslab_cap + 1);
+  P0.a_grid = launch_grid(e->tile_n[0]);
+  P1.a_grid = launch_grid(e->tile_n[1]);
+  P0.a_lds = e->a_lds[0];
+  P1.a_lds = e->a_lds[1];
+  P0.a_slab = e->a_slab[0];
+  P1.a_slab = e->a_slab[1];
   const SolveParams sp = e->params();
   const int nk = 3 * e->cfg.steps_per_launch;
   const bool f64 = e->fp64, dm = e->dict_mode;
@@ -945,7 +1047,8 @@ static int run_steps(mm_engine *e)
   const int grid0 = launch_grid(e->act_n[0]), grid1 = launch_grid(e->act_n[1]);
   if (grid0 == 0 && grid1 == 0) return 0;
   if (!e->cfg.use_graph) return queue_steps(e, grid0, grid1);
-  const std::pair<int, int> key(grid0, grid1);
+  const std::array<int, 8> key = {grid0, grid1, launch_grid(e->tile_n[0]), launch_grid(e->tile_n[1]),
+                                  e->a_lds[0], e->a_lds[1], e->a_slab[0], e->a_slab[1]};
   auto it = e->graphs.find(key);
   if (it == e->graphs.end()) {
     MM_HIP(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
@@ -1319,11 +1422,19 @@ static int time_kernel(mm_engine *e, int what, int reps, double *avg_ms, double 
   const int grid = solver_grid(e, hw);
   EngPtrs P = e->ptrs();
   double rows = 0.0, nrve = 0.0;
+  int lds = 0, slab_mode = 0;
   for (int s = 0; s < e->rve_cap; s++)
     if (e->slots[s].state != SL_FREE) {
       rows += e->slots[s].nrow;
       nrve += 1.0;
+      const int need = march_lds_need(e, e->slots[s]);
+      lds = std::max(lds, need);
+      if (need == 0) slab_mode = 1;
     }
+  if (e->cfg.march_planes > 0 && !e->fp64 && lds > 0) {
+    P.a_lds = march_lds_class(lds);
+    P.a_slab = slab_mode;
+  }
   hipEvent_t a, b;
   MM_HIP(hipEventCreate(&a));
   MM_HIP(hipEventCreate(&b));

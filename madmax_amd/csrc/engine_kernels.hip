@@ -54,6 +54,11 @@ template <int K>
 struct RedLds {
   static constexpr int size = 16 * ((4 * K > 12) ? 4 * K : 12);
 };
+// The same for the plane march, whose group sums go four values at a time.
+template <int K>
+struct MarchLds {
+  static constexpr int size = 16 * ((K > 4) ? K : 4);
+};
 
 
 // MI355X dispatches workgroup i to XCD i % 8.  The solver grids are a
@@ -72,16 +77,17 @@ __device__ __forceinline__ int xcd_slab()
 // covers the list.  The list is split in 8 contiguous parts, one per XCD, so
 // neighbouring slabs of an RVE share an L2.  Returns false for workgroups
 // beyond the list.
-__device__ __forceinline__ bool active_slab(const EngPtrs &P, int &slab)
+__device__ __forceinline__ bool active_in(const int *list, int &slab)
 {
-  const int n = P.act_list[0];
+  const int n = list[0];
   const int per = (n + kXcd - 1) / kXcd;
   const int j = blockIdx.x / kXcd;
   const int i = (blockIdx.x % kXcd) * per + j;
   if (j >= per || i >= n) return false;
-  slab = P.act_list[1 + i];
+  slab = list[1 + i];
   return true;
 }
+__device__ __forceinline__ bool active_slab(const EngPtrs &P, int &slab) { return active_in(P.act_list, slab); }
 
 __device__ __forceinline__ int node_class(int ix, int iy, int iz, int nx, int ny, int nz)
 {
@@ -874,19 +880,14 @@ __device__ bool inner_check(SolveState &N, const double (&s2)[2 * K], const Solv
 
 // Volume averages and tangent of a finished RVE (micro/src/homogenize.c:
 // 344-368, :80) from the averaging partials; run by one workgroup.
-template <int K>
+template <int K, int L>
 __device__ void finish_rve(const EngPtrs &P, int slot, const RveDev *R, SolveState &N,
-                           const SolveParams &sp, double (&lds)[RedLds<K>::size])
+                           const SolveParams &sp, double (&lds)[L])
 {
   __shared__ double tot[MM_MAX_RHS][12];
   const double *p0 = slab_part(P, R->first_slab).p;
 #pragma unroll 1
-  for (int k = 0; k < K; k++) {
-    double s12[12];
-    group_sum_cols<12>(p0, R->nslab, P.part_stride, k * 12, s12, lds);
-    if (threadIdx.x == 0)
-      for (int i = 0; i < 12; i++) tot[k][i] = s12[i];
-  }
+  for (int k = 0; k < K; k++) group_sum_cols_lds<12>(p0, R->nslab, P.part_stride, k * 12, &tot[k][0], lds);
   __shared__ double sig[MM_MAX_RHS][6];
   if (threadIdx.x != 0) return;
   RveOut *o = P.out + slot;
@@ -935,9 +936,11 @@ __device__ void finish_rve(const EngPtrs &P, int slot, const RveDev *R, SolveSta
 // branch between the gathers would serialize their loads); the cases in
 // `skip` have met their sweep target and skip only the update.  The own z
 // rows come from the gather, not a second read.
+// dc: the workgroup's dictionary cache, filled (dictionary storage only).
 template <typename TS, int K, bool DICT>
-__device__ __forceinline__ void inner_a(const EngPtrs &P, const RveDev *R, int slot, int slab, unsigned skip,
-                                        const double *beta, double (&lds)[RedLds<K>::size])
+__device__ __forceinline__ void inner_a_dc(const EngPtrs &P, const RveDev *R, int slot, int slab, unsigned skip,
+                                           const double *beta, double (&lds)[RedLds<K>::size],
+                                           const DictCache<TS> *dc)
 {
   const int tid = threadIdx.x, RS = 3 * P.SN;
   const int n = P.slab_node0[slab] + tid;
@@ -945,7 +948,6 @@ __device__ __forceinline__ void inner_a(const EngPtrs &P, const RveDev *R, int s
   double v[K];
 #pragma unroll
   for (int k = 0; k < K; k++) v[k] = 0.0;
-  const DictCache<TS> *dc = dict_cache<TS, DICT>(P, R, slot, slab);
   if (n < R->nrow / 3) {
     TS w[3][K], zc[3][K];
     slab_node_spmv<TS, K, DICT>(P, R, slot, slab, tid, n, dc, w, zc);
@@ -993,6 +995,13 @@ __device__ __forceinline__ void inner_a(const EngPtrs &P, const RveDev *R, int s
   }
 }
 
+template <typename TS, int K, bool DICT>
+__device__ __forceinline__ void inner_a(const EngPtrs &P, const RveDev *R, int slot, int slab, unsigned skip,
+                                        const double *beta, double (&lds)[RedLds<K>::size])
+{
+  inner_a_dc<TS, K, DICT>(P, R, slot, slab, skip, beta, lds, dict_cache<TS, DICT>(P, R, slot, slab));
+}
+
 // ---------------------------------------------------------------------------
 // Kernel A (reads st[2], writes st[0]).
 // PH_RESRED: starts the sweep from the residual norms, then continues as
@@ -1014,6 +1023,7 @@ __global__ __launch_bounds__(256) void k_step_a(EngPtrs P, SolveParams sp)
   const int slot = P.slab_rve[slab];
   if (slot < 0) return;
   RveDev *R = P.rves + slot;
+  if (P.a_lds > 0 && R->mcpp > 0) return;  // k_step_a_march's
   const bool writer = slab == R->first_slab;
   const int nslab = R->nslab;
   const int phase0 = R->st[2].phase;
@@ -1059,6 +1069,498 @@ __global__ __launch_bounds__(256) void k_step_a(EngPtrs P, SolveParams sp)
 
   // PH_INNER
   inner_a<TS, K, DICT>(P, R, slot, slab, N.done_in, N.beta, lds);
+}
+
+
+// ---------------------------------------------------------------------------
+// Kernel A as a plane march.  In the slab kernel every lane gathers its 27
+// neighbours' z through the vector memory path, 27 accesses per node and load
+// case, which bounds the kernel by address processing rather than by HBM.
+// Here the workgroup of tile t takes an in-plane chunk of SN nodes (in-plane
+// index j = c SN + lane, node z nxy + j) through a run of mL z-planes and
+// keeps, for three consecutive planes, the window of z around the chunk
+// (in-plane indices c SN - nx - 1 .. c SN + SN + nx: every in-plane neighbour
+// of the chunk) in an LDS ring, one row-major run per plane, load case and
+// direction.  A node's 27 neighbours are then LDS reads at fixed offsets, and
+// z is read from memory about once per launch.  The plane two ahead is loaded
+// into registers while the current plane is computed and stored into the
+// ring slot that the plane behind frees.  Load cases go in groups of
+// march_cases(sizeof(TS)) to bound the ring, each group a tile of its own
+// (more, shorter tiles: a lightly loaded GPU still fills), and a tile whose
+// cases have all met their sweep target returns at once.  w, the p/q update
+// and the partial p.q of a node are inner_a's arithmetic (the SpMV in the
+// same fma order, so the same bits); block b's p.q partials of the group's
+// cases go to slab first_slab + b and the RVE's slabs beyond its blocks
+// hold 0.  RVEs with mcpp == 0 (a window wider than
+// two positions per lane) stay with k_step_a, which skips the others.
+
+struct MarchGeo {
+  int nx, ny, nz, nxy, W;   // RVE nodes per axis and per plane; window length
+  int z0, z1;               // the tile's planes
+  int wb;                   // in-plane index of window position 0
+  int j, pos0;              // the lane's in-plane node and its window position
+  bool lane_ok, plane_int;  // j is in the chunk; j is off the four side faces
+};
+
+// Tile t = chunk c + mcpp * run r: the chunks split a plane's nxy nodes and
+// the runs the nz planes as evenly as integers allow (at most SN nodes and
+// mL planes each), so that the tiles of an RVE cost the same.
+__device__ __forceinline__ MarchGeo march_geo(const EngPtrs &P, const RveDev *R, int t)
+{
+  MarchGeo g;
+  g.nx = R->nx;
+  g.ny = R->ny;
+  g.nz = R->nz;
+  g.nxy = g.nx * g.ny;
+  const int c = t % R->mcpp, zr = t / R->mcpp;
+  const int c0 = c * g.nxy / R->mcpp, c1 = (c + 1) * g.nxy / R->mcpp;
+  g.z0 = zr * g.nz / R->mnzc;
+  g.z1 = (zr + 1) * g.nz / R->mnzc;
+  g.W = (c1 - c0) + 2 * g.nx + 2;
+  g.wb = c0 - g.nx - 1;
+  g.j = c0 + (int)threadIdx.x;
+  g.lane_ok = g.j < c1;
+  g.pos0 = (g.lane_ok ? (int)threadIdx.x : 0) + g.nx + 1;  // idle lanes read a valid position
+  const int ix = g.j % g.nx, iy = g.j / g.nx;
+  g.plane_int = g.lane_ok && ix > 0 && ix < g.nx - 1 && iy > 0 && iy < g.ny - 1;
+  return g;
+}
+
+// Registers <- plane zz of load cases g0 .. g0+GR-1 at the lane's window
+// positions lane and lane + SN (the window is at most 2 SN long).  Positions
+// off the RVE's plane, and planes outside the RVE, are 0: no node reads them.
+template <typename TS, int K, int GR>
+__device__ __forceinline__ void march_fetch(const EngPtrs &P, __amdgpu_buffer_rsrc_t zr, const MarchGeo &g, int zz,
+                                            int g0, TS (&pf)[2][GR][3])
+{
+  const int RS = 3 * P.SN;
+#pragma unroll
+  for (int i = 0; i < 2; i++) {
+#pragma unroll
+    for (int gk = 0; gk < GR; gk++)
+#pragma unroll
+      for (int d = 0; d < 3; d++) pf[i][gk][d] = (TS)0;
+    const int pos = (int)threadIdx.x + i * P.SN, jj = g.wb + pos;
+    if (zz >= 0 && zz < g.nz && pos < g.W && jj >= 0 && jj < g.nxy) {
+      const int m = zz * g.nxy + jj;
+      const unsigned off = (unsigned)(((m >> P.sn_shift) * K * RS + 3 * (m & (P.SN - 1))) * (int)sizeof(TS));
+#pragma unroll
+      for (int gk = 0; gk < GR; gk++)
+        buffer_load3(zr, off, (unsigned)((g0 + gk) * RS * (int)sizeof(TS)), pf[i][gk]);
+    }
+  }
+}
+
+// Ring slot s <- the registers of march_fetch.
+template <typename TS, int GR>
+__device__ __forceinline__ void march_put(TS *ring, const MarchGeo &g, int s, const TS (&pf)[2][GR][3])
+{
+#pragma unroll
+  for (int i = 0; i < 2; i++) {
+    const int pos = (int)threadIdx.x + i * (int)blockDim.x;
+    if (pos < g.W)
+#pragma unroll
+      for (int gk = 0; gk < GR; gk++)
+#pragma unroll
+        for (int d = 0; d < 3; d++) ring[((s * GR + gk) * 3 + d) * g.W + pos] = pf[i][gk][d];
+  }
+}
+
+// The dictionary entries of the patterns a tile's interior nodes use, the
+// first kMarchDictCap of them in pattern order, copied to LDS (dv) once per
+// tile; slot_of[p] is the LDS entry of pattern p or -1.  Run by every thread.
+template <typename TS>
+__device__ void march_dict_fill(const EngPtrs &P, const RveDev *R, int slot, int fs, const MarchGeo &g, TS *dv,
+                                signed char *slot_of)
+{
+  __shared__ int s_pat[kMarchDictCap];
+  __shared__ int s_cnt;
+  const int tid = threadIdx.x;
+  for (int i = tid; i < kDictPat; i += blockDim.x) slot_of[i] = 0;
+  __syncthreads();
+  if (g.plane_int)
+    for (int z = max(g.z0, 1); z < min(g.z1, g.nz - 1); z++) {
+      const int n = z * g.nxy + g.j;
+      slot_of[P.npat[(long long)(fs + (n >> P.sn_shift)) * P.SN + (n & (P.SN - 1))]] = 1;  // same value from every writer
+    }
+  __syncthreads();
+  if (tid < 64) {  // the first wavefront numbers the patterns in use
+    int base = 0;
+    for (int c = 0; c < kDictPat; c += 64) {
+      const bool used = slot_of[c + tid] != 0;
+      const unsigned long long m = __ballot(used);
+      const int sidx = base + __popcll(m & ((1ull << tid) - 1ull));
+      int v = -1;
+      if (used && sidx < kMarchDictCap) {
+        v = sidx;
+        s_pat[sidx] = c + tid;
+      }
+      slot_of[c + tid] = (signed char)v;
+      base += __popcll(m);
+    }
+    if (tid == 0) s_cnt = min(base, kMarchDictCap);
+  }
+  __syncthreads();
+  const TS *D0 = (const TS *)P.dict + (size_t)slot * kDictPat * kDictEntry;
+  for (int e = tid; e < s_cnt * kDictEntry; e += blockDim.x) {
+    const int sidx = e / kDictEntry, off = e - sidx * kDictEntry;
+    dv[sidx * kDictEntry + off] = D0[(size_t)s_pat[sidx] * kDictEntry + off];
+  }
+  __syncthreads();
+}
+
+// w += the 27-block product of an interior node from dictionary entry D (a
+// wave-uniform global address, the tile's LDS copy, or a per-lane global
+// address: SRC), the neighbours read from the ring in key order.
+template <typename TS, int GR, int SRC>
+__device__ __forceinline__ void march_rows(const TS *D, const TS *ring, const MarchGeo &g, int z, TS (&w)[3][GR])
+{
+  const int W = g.W;
+#pragma unroll 1
+  for (int dz = 0; dz < 3; dz++) {
+    const int s = (z + dz + 2) % 3;  // ring slot of plane z + dz - 1
+#pragma unroll 1
+    for (int dy = 0; dy < 3; dy++) {
+      const TS *rb = ring + s * GR * 3 * W + g.pos0 + (dy - 1) * g.nx - 1;
+      const TS *Db = D + (3 * dy + 9 * dz) * kDictBlk;
+#pragma unroll
+      for (int dx = 0; dx < 3; dx++) {
+        TS v[3][4];  // [d'][d]
+        if (SRC == DICT_SCALAR) {
+#pragma unroll
+          for (int dd = 0; dd < 3; dd++)
+#pragma unroll
+            for (int d = 0; d < 3; d++) v[dd][d] = uniform_load(Db + dx * kDictBlk + dd * 4 + d);
+        } else if (SRC == DICT_LDS) {
+#pragma unroll
+          for (int dd = 0; dd < 3; dd++) lds_row4(Db + dx * kDictBlk + dd * 4, v[dd]);
+        } else {
+#pragma unroll
+          for (int dd = 0; dd < 3; dd++) load_vec<TS, 4>(Db + dx * kDictBlk + dd * 4, v[dd]);
+        }
+#pragma unroll
+        for (int gk = 0; gk < GR; gk++) {
+          TS x[3];
+#pragma unroll
+          for (int dd = 0; dd < 3; dd++) x[dd] = rb[(gk * 3 + dd) * W + dx];
+#pragma unroll
+          for (int dd = 0; dd < 3; dd++)
+#pragma unroll
+            for (int d = 0; d < 3; d++) w[d][gk] = fma(v[dd][d], x[dd], w[d][gk]);
+        }
+      }
+    }
+  }
+}
+
+// Pattern byte of the lane's node in plane z (0 off the interior).
+__device__ __forceinline__ int march_pat(const EngPtrs &P, const MarchGeo &g, int fs, int z)
+{
+  if (!(g.plane_int && z > 0 && z < g.nz - 1)) return 0;
+  const int n = z * g.nxy + g.j;
+  return P.npat[(long long)(fs + (n >> P.sn_shift)) * P.SN + (n & (P.SN - 1))];
+}
+
+// One plane z of the tile for load cases g0 .. g0+GR-1: w = A z from the
+// ring, then the node's p/q update and its p.q into v (as inner_a).  pat is
+// the node's pattern byte (dictionary storage), loaded one plane ahead: the
+// load of the next plane's byte is issued after this plane's p/q loads, so
+// that waiting for it never waits for them (vector loads retire in order).
+template <typename TS, int K, int GR, bool DICT>
+__device__ __forceinline__ void march_plane(const EngPtrs &P, int slot, int fs, const MarchGeo &g, const TS *ring,
+                                            const TS *dv, const signed char *slot_of, int z, int pat, int &pat_next,
+                                            int g0, unsigned skip, const double *beta, double (&v)[K])
+{
+  const int SN = P.SN, RS = 3 * SN, W = g.W;
+  const int n = z * g.nxy + g.j;
+  const int sl = fs + (n >> P.sn_shift), tl = n & (SN - 1);  // the node's slab and lane there
+  const bool interior = g.plane_int && z > 0 && z < g.nz - 1;
+  const long long nb = (long long)sl * K * RS + 3 * tl;
+  TS *__restrict__ p = (TS *)P.p;
+  TS *__restrict__ q = (TS *)P.q;
+  // the node's p and q rows, loaded first so that they arrive during the SpMV
+  TS pv[3][GR], qv[3][GR];
+  if (g.lane_ok)
+#pragma unroll
+    for (int gk = 0; gk < GR; gk++) {
+      if ((skip >> (g0 + gk)) & 1u) continue;
+      const long long i = nb + (long long)(g0 + gk) * RS;
+      TS a[3], c[3];
+      nt_load3(p + i, a);
+      nt_load3(q + i, c);
+#pragma unroll
+      for (int d = 0; d < 3; d++) {
+        pv[d][gk] = a[d];
+        qv[d][gk] = c[d];
+      }
+    }
+  if (DICT && z + 1 < g.z1) pat_next = march_pat(P, g, fs, z + 1);
+  TS w[3][GR], zc[3][GR];
+  const int sc = z % 3;
+#pragma unroll
+  for (int gk = 0; gk < GR; gk++)
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+      zc[d][gk] = ring[((sc * GR + gk) * 3 + d) * W + g.pos0];
+      w[d][gk] = (TS)0;
+    }
+  if constexpr (DICT) {
+    // interior lanes: their dictionary entry, through the scalar unit when
+    // the wavefront's interior lanes share one pattern, else from the tile's
+    // LDS copy, else (a pattern beyond the copy) per lane through L1
+    const unsigned long long im = __ballot(interior);
+    if (im != 0) {
+      const TS *D0 = (const TS *)P.dict + (size_t)slot * kDictPat * kDictEntry;
+      const int pat0 = __builtin_amdgcn_readlane(pat, (int)__builtin_ctzll(im));
+      const int cs = interior ? (int)slot_of[pat] : 0;
+      if (__all(!interior || pat == pat0)) {
+        if (interior) march_rows<TS, GR, DICT_SCALAR>(D0 + (size_t)pat0 * kDictEntry, ring, g, z, w);
+      } else if (__all(!interior || cs >= 0)) {
+        if (interior) march_rows<TS, GR, DICT_LDS>(dv + cs * kDictEntry, ring, g, z, w);
+      } else {
+        if (interior) march_rows<TS, GR, DICT_GLOBAL>(D0 + (size_t)pat * kDictEntry, ring, g, z, w);
+      }
+    }
+    if (!interior)
+#pragma unroll
+      for (int gk = 0; gk < GR; gk++)
+#pragma unroll
+        for (int d = 0; d < 3; d++) w[d][gk] = zc[d][gk] + (TS)0;
+  } else if (g.lane_ok) {
+    // full storage: the node's block columns and values; a stored neighbour
+    // is the structured one of its key, which sits in the ring
+    const int *__restrict__ bc = P.bcols + (long long)sl * 27 * SN + tl;
+    const TS *__restrict__ vr = (const TS *)P.vals + (long long)sl * kW * RS + 3 * tl;
+#pragma unroll 1
+    for (int dz = 0; dz < 3; dz++) {
+      const int s = (z + dz + 2) % 3;
+#pragma unroll
+      for (int dy = 0; dy < 3; dy++) {
+        const int pb = g.pos0 + (dy - 1) * g.nx - 1;
+#pragma unroll
+        for (int dx = 0; dx < 3; dx++) {
+          const int key = dx + 3 * dy + 9 * dz;
+          const int m = __builtin_nontemporal_load(bc + (long long)key * SN);
+          TS vb[3][3];
+#pragma unroll
+          for (int dd = 0; dd < 3; dd++)
+#pragma unroll
+            for (int d = 0; d < 3; d++) vb[dd][d] = __builtin_nontemporal_load(vr + (long long)(3 * key + dd) * RS + d);
+          if (m >= 0)
+#pragma unroll
+            for (int gk = 0; gk < GR; gk++) {
+              TS x[3];
+#pragma unroll
+              for (int dd = 0; dd < 3; dd++) x[dd] = ring[((s * GR + gk) * 3 + dd) * W + pb + dx];
+#pragma unroll
+              for (int dd = 0; dd < 3; dd++)
+#pragma unroll
+                for (int d = 0; d < 3; d++) w[d][gk] = fma(vb[dd][d], x[dd], w[d][gk]);
+            }
+        }
+      }
+    }
+  }
+  if (!g.lane_ok) return;
+#pragma unroll
+  for (int gk = 0; gk < GR; gk++) {
+    const int k = g0 + gk;
+    if ((skip >> k) & 1u) continue;
+    const TS b = (TS)beta[k];
+    const long long i = nb + (long long)k * RS;
+    TS po[3], qo[3];
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+      po[d] = fma(b, pv[d][gk], zc[d][gk]);
+      qo[d] = fma(b, qv[d][gk], w[d][gk]);
+      v[k] = fma((double)po[d], (double)qo[d], v[k]);
+    }
+    nt_store3(p + i, po);
+    nt_store3(q + i, qo);
+  }
+}
+
+// One round of load cases g0 .. g0+GR-1 over the tile's planes.  Every lane
+// of the workgroup runs it (barriers inside).  The prologue's three planes
+// and the first pattern bytes are loaded before the dictionary cache is
+// filled (dictionary storage), so that their memory latency overlaps it.
+template <typename TS, int K, int GR, bool DICT>
+__device__ __forceinline__ void march_round(const EngPtrs &P, const RveDev *R, int slot, int fs, const MarchGeo &g,
+                                            TS *ring, TS *dv, signed char *slot_of, __amdgpu_buffer_rsrc_t zr,
+                                            int g0, unsigned skip, const double *beta, double (&v)[K])
+{
+  // prologue: planes z0 - 1, z0, z0 + 1 (slot = plane % 3; plane -1 unused)
+  TS p0[2][GR][3], p1[2][GR][3], pf[2][GR][3];
+  march_fetch<TS, K, GR>(P, zr, g, g.z0 - 1, g0, p0);
+  march_fetch<TS, K, GR>(P, zr, g, g.z0, g0, p1);
+  march_fetch<TS, K, GR>(P, zr, g, g.z0 + 1, g0, pf);
+  int pat = DICT ? march_pat(P, g, fs, g.z0) : 0;
+  if constexpr (DICT) march_dict_fill<TS>(P, R, slot, fs, g, dv, slot_of);
+  march_put<TS, GR>(ring, g, (g.z0 + 2) % 3, p0);
+  march_put<TS, GR>(ring, g, g.z0 % 3, p1);
+  march_put<TS, GR>(ring, g, (g.z0 + 1) % 3, pf);
+  __syncthreads();
+  const int zlast = min(g.z1, g.nz - 1);  // the last plane the tile reads
+#pragma unroll 1
+  for (int z = g.z0; z < g.z1; z++) {
+    const bool pre = z + 2 <= zlast;
+    if (pre) march_fetch<TS, K, GR>(P, zr, g, z + 2, g0, pf);
+    int pat_next = 0;
+    march_plane<TS, K, GR, DICT>(P, slot, fs, g, ring, dv, slot_of, z, pat, pat_next, g0, skip, beta, v);
+    pat = pat_next;
+    if (pre) {
+      __syncthreads();  // plane z - 1 read by every lane
+      march_put<TS, GR>(ring, g, (z + 2) % 3, pf);
+      __syncthreads();
+    }
+  }
+  __syncthreads();  // the ring is free for the next round
+}
+
+// The round of case group cg (cases G0 .. G0+GR-1 for the group whose G0 is
+// cg * march_cases).
+template <typename TS, int K, bool DICT, int G0>
+__device__ __forceinline__ void march_group(const EngPtrs &P, const RveDev *R, int slot, int fs, const MarchGeo &g,
+                                            TS *ring, TS *dv, signed char *slot_of, __amdgpu_buffer_rsrc_t zr,
+                                            int cg, unsigned skip, const double *beta, double (&v)[K])
+{
+  constexpr int GM = march_cases((int)sizeof(TS));
+  constexpr int GR = (K - G0 < GM) ? K - G0 : GM;
+  if (cg == G0 / GM) march_round<TS, K, GR, DICT>(P, R, slot, fs, g, ring, dv, slot_of, zr, G0, skip, beta, v);
+  if constexpr (G0 + GR < K)
+    march_group<TS, K, DICT, G0 + GR>(P, R, slot, fs, g, ring, dv, slot_of, zr, cg, skip, beta, v);
+}
+
+extern __shared__ __attribute__((aligned(16))) double march_dyn[];
+
+// The INNER work of tile t (block t % mnt, case group t / mnt) of an RVE
+// with mcpp > 0; the dictionary cache and the LDS ring are the launch's
+// dynamic LDS.
+template <typename TS, int K, bool DICT>
+__device__ void march_tile(const EngPtrs &P, const RveDev *R, int slot, int t, unsigned skip, const double *beta,
+                           double (&lds)[MarchLds<K>::size])
+{
+  constexpr int GM = march_cases((int)sizeof(TS));
+  const int b = t % R->mnt, cg = t / R->mnt;
+  const int k0 = cg * GM, k1 = min(K, k0 + GM);
+  const unsigned gmask = ((1u << k1) - 1u) & ~((1u << k0) - 1u);
+  if (t == 0) {  // the RVE's slabs without a block contribute 0 to p.q
+    const int rest = R->nslab - R->mnt;
+    for (int i = threadIdx.x; i < rest * K; i += blockDim.x) {
+      const int k = i / rest, sb = R->mnt + i % rest;
+      slab_part(P, R->first_slab + sb)[k] = 0.0;
+    }
+  }
+  const int slab = R->first_slab + b;
+  if ((skip & gmask) == gmask) {  // every case of the group is done
+    if (threadIdx.x == 0)
+      for (int k = k0; k < k1; k++) slab_part(P, slab)[k] = 0.0;
+    return;
+  }
+  const int RS = 3 * P.SN;
+  const MarchGeo g = march_geo(P, R, b);
+  const int fs = __builtin_amdgcn_readfirstlane(R->first_slab), ns = __builtin_amdgcn_readfirstlane(R->nslab);
+  const __amdgpu_buffer_rsrc_t zr =
+      rve_buffer((const TS *)P.z + (long long)fs * K * RS, ns * K * RS * (int)sizeof(TS));
+  // dynamic LDS: the dictionary cache (dictionary storage), then the ring
+  unsigned char *lds0 = reinterpret_cast<unsigned char *>(march_dyn);
+  TS *dv = reinterpret_cast<TS *>(lds0);
+  signed char *slot_of = reinterpret_cast<signed char *>(lds0 + kMarchDictCap * kDictEntry * sizeof(TS));
+  TS *ring = reinterpret_cast<TS *>(lds0 + (DICT ? march_dict_bytes((int)sizeof(TS)) : 0));
+  double v[K];
+#pragma unroll
+  for (int k = 0; k < K; k++) v[k] = 0.0;
+  march_group<TS, K, DICT, 0>(P, R, slot, fs, g, ring, dv, slot_of, zr, cg, skip, beta, v);
+  block_sum<K>(v, lds);
+  if (threadIdx.x == 0) {
+    const SlabPart part = slab_part(P, slab);
+    for (int k = k0; k < k1; k++) part[k] = v[k];
+  }
+}
+
+// Kernel A over the tile list act_tile (the phase handling of k_step_a; the
+// workgroup of tile 0 writes the RVE's state, and the x64 += d of a sweep end
+// covers slabs t, t + mnt, ...).
+template <typename TS, int K, bool DICT>
+__device__ __forceinline__ void step_a_march(const EngPtrs &P, const SolveParams &sp)
+{
+  __shared__ double lds[MarchLds<K>::size];
+  __shared__ SolveState N;
+  __shared__ int s_xupd;
+  int slab;
+  if (!active_in(P.act_tile, slab)) return;
+  const int tid = threadIdx.x, RS = 3 * P.SN;
+  const int slot = P.slab_rve[slab];
+  if (slot < 0) return;
+  RveDev *R = P.rves + slot;
+  const int t = slab - R->first_slab;
+  if (R->mcpp == 0) return;  // not listed: k_step_a's
+  const bool writer = t == 0;
+  const int nslab = R->nslab;
+  const int phase0 = R->st[2].phase;
+  const double *p0 = slab_part(P, R->first_slab).p;
+  if (phase0 == PH_AVGRED) {
+    if (!writer) return;
+    if (tid == 0) N = R->st[2];
+    __syncthreads();
+    finish_rve<K>(P, slot, R, N, sp, lds);
+    if (tid == 0) R->st[0] = N;
+    return;
+  }
+  if (tid == 0) {
+    N = R->st[2];
+    s_xupd = 0;
+  }
+  // the totals of group_sum_cols; the sweep start's 4K values four at a time
+  // (fewer live registers), the every-iteration 2K in one pass (one memory
+  // round trip per tile)
+  if (phase0 == PH_RESRED) {
+    __shared__ double tot[4 * K];
+    group_sum_cols_lds<4 * K>(p0, nslab, P.part_stride, 3 * K, tot, lds);
+    if (tid == 0) start_sweep<K>(N, *reinterpret_cast<const double(*)[4 * K]>(tot), R, sp);
+  } else if (phase0 == PH_INNER) {
+    // scratch: the dynamic LDS, which the march has not started to use
+    double(&plds)[16 * 2 * K] = *reinterpret_cast<double(*)[16 * 2 * K]>(march_dyn);
+    double s2[2 * K];
+    group_sum_cols<2 * K>(p0, nslab, P.part_stride, K, s2, plds);
+    if (tid == 0) s_xupd = inner_check<K>(N, s2, sp) ? 1 : 0;
+  }
+  __syncthreads();
+  if (writer && tid == 0) R->st[0] = N;
+  if (N.phase != PH_INNER && !s_xupd) return;
+
+  if (s_xupd) {
+    const TS *d = (const TS *)P.d;
+    for (int b = t; b < nslab; b += R->mnt * R->mncg) {
+      const int sl = R->first_slab + b;
+      const int rows = min(RS, R->nrow - 3 * P.slab_node0[sl]);
+      const long long sb = (long long)sl * K * RS;
+#pragma unroll
+      for (int k = 0; k < K; k++) {
+        const long long o = sb + (long long)k * RS;
+        for (int e = tid; e < rows; e += blockDim.x) P.x64[o + e] += (double)d[o + e];
+      }
+    }
+    return;
+  }
+
+  // PH_INNER
+  march_tile<TS, K, DICT>(P, R, slot, t, N.done_in, N.beta, lds);
+}
+
+// Dictionary storage: bounded to 128 VGPRs (four waves per SIMD, what the
+// LDS of four workgroups per CU allows) without spills.  Full storage keeps
+// its block values in registers and is left unbounded.
+template <typename TS, int K>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_step_a_march_dict(EngPtrs P,
+                                                                                                  SolveParams sp)
+{
+  step_a_march<TS, K, true>(P, sp);
+}
+template <typename TS, int K>
+__global__ __launch_bounds__(256) void k_step_a_march_full(EngPtrs P, SolveParams sp)
+{
+  step_a_march<TS, K, false>(P, sp);
 }
 
 // ---------------------------------------------------------------------------
@@ -1366,9 +1868,28 @@ __global__ __launch_bounds__(256) void k_inner_time(EngPtrs P)
   const int slab = xcd_slab(), tid = threadIdx.x;
   const int slot = P.slab_rve[slab];
   if (slot < 0) return;
+  if (P.a_lds > 0 && P.rves[slot].mcpp > 0) return;  // k_march_time's
   if (tid < K) beta[tid] = 0.5;
   __syncthreads();
   inner_a<TS, K, DICT>(P, P.rves + slot, slot, slab, 0u, beta, lds);
+}
+
+// The same replica for the plane march (march_tile over the tiles of every
+// resident RVE with mcpp > 0); the launch's dynamic LDS covers them all.
+template <typename TS, int K, bool DICT>
+__global__ __launch_bounds__(256) void k_march_time(EngPtrs P)
+{
+  __shared__ double lds[MarchLds<K>::size];
+  __shared__ double beta[K];
+  const int slab = xcd_slab(), tid = threadIdx.x;
+  const int slot = P.slab_rve[slab];
+  if (slot < 0) return;
+  const RveDev *R = P.rves + slot;
+  const int t = slab - R->first_slab;
+  if (R->mcpp == 0 || t >= R->mnt * R->mncg) return;  // slab by slab: k_inner_time
+  if (tid < K) beta[tid] = 0.5;
+  __syncthreads();
+  march_tile<TS, K, DICT>(P, R, slot, t, 0u, beta, lds);
 }
 
 // ---------------------------------------------------------------------------
@@ -1378,11 +1899,20 @@ __global__ __launch_bounds__(256) void k_inner_time(EngPtrs P)
 template <typename TS, int K, bool DICT>
 static int launch_steps(const EngPtrs &P, const SolveParams &sp, int grid, int first, int nkern, hipStream_t s)
 {
-  if (grid % kXcd) return MM_ERR_ARG;  // the XCD-aware slab order needs whole groups of 8
+  if (grid % kXcd || P.a_grid % kXcd) return MM_ERR_ARG;  // the XCD-aware order needs whole groups of 8
+  if (P.a_lds < 0 || P.a_lds > kMarchMaxLds) return MM_ERR_ARG;
   if (first < 0 || first > 2 || nkern < 0) return MM_ERR_ARG;
   for (int i = first; i < first + nkern; i++) {
     switch (i % 3) {
-      case 0: k_step_a<TS, K, DICT><<<grid, P.SN, 0, s>>>(P, sp); break;
+      case 0:
+        // with the march on (a_lds > 0), its tiles and, if any RVE is
+        // processed slab by slab (a_slab), k_step_a for those
+        if (P.a_lds > 0 && P.a_grid > 0) {
+          if (DICT) k_step_a_march_dict<TS, K><<<P.a_grid, P.SN, P.a_lds, s>>>(P, sp);
+          else k_step_a_march_full<TS, K><<<P.a_grid, P.SN, P.a_lds, s>>>(P, sp);
+        }
+        if (P.a_lds == 0 || P.a_slab) k_step_a<TS, K, DICT><<<grid, P.SN, 0, s>>>(P, sp);
+        break;
       case 1: k_step_b<TS, K><<<grid, P.SN, 0, s>>>(P, sp); break;
       default: k_step_r<TS, K><<<grid, P.SN, 0, s>>>(P, sp); break;
     }
@@ -1406,8 +1936,12 @@ static int launch_assemble(const EngPtrs &P, int nblk, const int *slab_list, int
 template <typename TS, int K, bool DICT>
 static int launch_time(const EngPtrs &P, int grid, int what, hipStream_t s)
 {
-  if (what == 1) k_inner_time<TS, K, DICT><<<grid, P.SN, 0, s>>>(P);
-  else k_spmv_time<TS, K, DICT><<<grid, P.SN, 0, s>>>(P);
+  if (what == 1) {
+    if (P.a_lds > 0) k_march_time<TS, K, DICT><<<grid, P.SN, P.a_lds, s>>>(P);
+    if (P.a_lds == 0 || P.a_slab) k_inner_time<TS, K, DICT><<<grid, P.SN, 0, s>>>(P);
+  } else {
+    k_spmv_time<TS, K, DICT><<<grid, P.SN, 0, s>>>(P);
+  }
   MM_HIP(hipGetLastError());
   return 0;
 }
@@ -1432,6 +1966,8 @@ static int launch_time(const EngPtrs &P, int grid, int what, hipStream_t s)
   } while (0)
 
 bool eng_supported_rhs(int K) { return K == 1 || K == 6 || K == 7; }
+
+
 
 int eng_launch_steps(bool fp64, int K, bool dict, const EngPtrs &P, const SolveParams &sp, int grid,
                      int first, int nkern, hipStream_t s)

@@ -52,6 +52,31 @@ __device__ inline void group_sum_cols(const double *p0, int nslab, long long vst
   block_sum<NV>(out, lds);
 }
 
+// group_sum_cols four values at a time, the totals written to out[] (LDS,
+// valid in every thread after the call); the same bits with fewer live
+// registers.
+template <int NV, int L>
+__device__ inline void group_sum_cols_lds(const double *p0, int nslab, long long vstride, int off, double *out,
+                                          double (&lds)[L])
+{
+#pragma unroll 1
+  for (int c = 0; c < NV; c += 4) {
+    double v[4];
+#pragma unroll
+    for (int i = 0; i < 4; i++) v[i] = 0.0;
+    for (int b = threadIdx.x; b < nslab; b += blockDim.x)
+#pragma unroll
+      for (int i = 0; i < 4; i++)
+        if (c + i < NV) v[i] += p0[(size_t)(off + c + i) * vstride + b];
+    block_sum<4>(v, lds);
+    if (threadIdx.x == 0)
+#pragma unroll
+      for (int i = 0; i < 4; i++)
+        if (c + i < NV) out[c + i] = v[i];
+  }
+  __syncthreads();
+}
+
 // The same over row-major partials: value i of slab b at p0[b * stride + i].
 template <int NV, int L>
 __device__ inline void group_sum_rows(const double *p0, int nslab, int stride, double (&out)[NV],

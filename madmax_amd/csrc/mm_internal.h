@@ -111,7 +111,13 @@ struct RveDev {
   double hx, hy, hz, vol;
   int nfib, tangent_mode, nload, warm;
   int affine;  // cold start from the affine field u = E X (else interior zero)
-  int pad_a;
+  // plane-march tiling of kernel A: the RVE's nodes as mnt = mcpp * mnzc
+  // blocks, mcpp in-plane chunks of at most SN nodes times mnzc runs of at
+  // most mL z-planes (even splits), each marched once per group of load
+  // cases (mncg groups of march_cases): tile t = block + mnt * group, t <
+  // mnt * mncg <= nslab, is processed by the workgroup of slab
+  // first_slab + t.  mcpp == 0: kernel A works slab by slab.
+  int mcpp, mL, mnzc, mnt, mncg;
   double fx[MM_MAX_FIBERS], fy[MM_MAX_FIBERS], fr[MM_MAX_FIBERS];
   double young[kNmat], poisson[kNmat];
   double load[MM_MAX_RHS][MM_NVOI];
@@ -156,9 +162,14 @@ struct EngPtrs {
   const int *act_list;  // [0]: number of slabs being solved, [1..]: those slabs
   const void *dict;     // [rve slot][256][27][12] interior 3-row ELL blocks (TS)
   const double *dict64; // the same in FP64 (the residual's matrix)
+  const int *act_tile;  // [0]: tiles of kernel A's plane march, [1..]: their slabs
   int SN;        // nodes per slab = lanes per workgroup (a power of two)
   int sn_shift;  // log2(SN)
+  int a_grid;    // grid of the plane march over act_tile (a_lds > 0)
+  int a_lds;     // dynamic LDS bytes of the plane march; 0: march off
+  int a_slab;    // march on: some RVEs (mcpp == 0) need k_step_a as well
 };
+
 
 // Both matrix storages keep a node's 27 neighbour blocks in neighbour-key
 // order, key = (dx+1) + 3(dy+1) + 9(dz+1) (z-major, x fastest), so the SpMV
@@ -171,6 +182,24 @@ struct EngPtrs {
 constexpr int kDictPat = 256;
 constexpr int kDictBlk = 12;
 constexpr int kDictEntry = 27 * kDictBlk;
+
+// Load cases staged in LDS per round of the plane march.  Two keep an FP32
+// march of a 30^3 RVE (window 318 nodes) with its dictionary cache under
+// 40 KB, four workgroups per CU; three cases per round fit only three.
+constexpr int march_cases(int) { return 2; }
+// Dictionary entries a march tile keeps in LDS, and the bytes of that cache
+// (entries, then one slot byte per pattern), a multiple of 16.
+constexpr int kMarchDictCap = 10;
+constexpr int march_dict_bytes(int tsz) { return (kMarchDictCap * kDictEntry * tsz + kDictPat + 15) / 16 * 16; }
+// LDS of the plane march: the dictionary cache (dictionary storage), then a
+// ring of three planes of the window (SN + 2 nx + 2 nodes) for the round's
+// load cases, three rows each.
+inline int march_lds_bytes(int SN, int nx, int K, int tsz, bool dict)
+{
+  const int g = K < march_cases(tsz) ? K : march_cases(tsz);
+  return (dict ? march_dict_bytes(tsz) : 0) + 3 * g * 3 * (SN + 2 * nx + 2) * tsz;
+}
+constexpr int kMarchMaxLds = 64 * 1024;
 
 // assembly modes
 enum AsmMode : int { ASM_FULL = 0, ASM_X_ONLY = 1, ASM_DICT = 2 };

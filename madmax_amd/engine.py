@@ -86,7 +86,7 @@ class Engine:
                  slab_nodes=0, rtol=1e-10, abs_tol=0.0, inner_rtol=1e-5, max_inner_its=20000,
                  max_sweeps=8, steps_per_launch=16, use_graph=True, sync_every_step=False,
                  device=0, ell_storage=_lib.MM_ELL_DICT, initial_guess=_lib.MM_GUESS_AFFINE,
-                 solver_streams=2, profile_step_kernel=False):
+                 solver_streams=2, profile_step_kernel=False, march_planes=None):
         L = lib()
         c = EngineConfig()
         check(L.mm_engine_config_default(C.byref(c)), "mm_engine_config_default")
@@ -108,6 +108,8 @@ class Engine:
         c.initial_guess = int(initial_guess)
         c.solver_streams = int(solver_streams)
         c.profile_step_kernel = int(bool(profile_step_kernel))
+        if march_planes is not None:
+            c.march_planes = int(march_planes)
         self.cfg = c
         self.nrhs = nrhs
         h = C.c_void_p()

@@ -97,6 +97,45 @@ def test_slab_shapes_agree():
         assert normwise_rel(b["c_tangent"], c["c_tangent"]) <= 1e-12
 
 
+def test_plane_march_matches_slab_kernel():
+    """Kernel A slab by slab (march_planes=0) and as a z-plane march with two
+    run lengths: every variant meets the oracle bar and they agree closely
+    (the SpMV bits are the same; only the grouping of the p.q partial sums
+    differs)."""
+    specs = [rve_spec(8, [(0.45, 0.5, 0.3)]), rve_spec((12, 9, 7), [(0.4, 0.6, 0.25)]),
+             configs.c2_spec(2, 5, n=14)]
+    refs = [_oracle_tangent(s) for s in specs]
+    out = {}
+    for planes in (0, 8, 2):
+        with Engine(nrhs=6, hbm_budget=GB, max_rves=16, march_planes=planes) as eng:
+            out[planes] = _solve_batch(eng, specs)
+        for s, r, c_ref in zip(specs, out[planes], refs):
+            assert r["status"] == 0
+            assert normwise_rel(r["c_tangent"], c_ref) <= TOL, (planes, s.nex)
+    for a, b, c in zip(out[0], out[8], out[2]):
+        assert normwise_rel(b["c_tangent"], a["c_tangent"]) <= 1e-12
+        assert normwise_rel(c["c_tangent"], a["c_tangent"]) <= 1e-12
+
+
+@pytest.mark.parametrize("storage", [MM_ELL_FULL, MM_ELL_DICT])
+def test_march_and_slab_rves_in_one_batch(storage):
+    """With 64 nodes per slab an RVE 34 elements wide cannot march (its window
+    exceeds two positions per lane) and goes slab by slab beside RVEs that
+    march: both kernels A run in the same step, and every RVE gets the bits it
+    gets alone."""
+    wide = rve_spec((34, 6, 5), [(0.5, 0.5, 0.3)])
+    batch = [rve_spec(9, [(0.45, 0.55, 0.3)]), wide, rve_spec((12, 10, 8), [(0.5, 0.4, 0.28)])]
+    with Engine(nrhs=6, hbm_budget=GB, max_rves=16, slab_nodes=64, ell_storage=storage) as eng:
+        mixed = _solve_batch(eng, batch)
+    for i, s in enumerate(batch):
+        with Engine(nrhs=6, hbm_budget=GB, max_rves=16, slab_nodes=64, ell_storage=storage) as eng:
+            alone = _solve_batch(eng, [s])[0]
+        assert mixed[i]["status"] == 0
+        assert np.array_equal(alone["c_tangent"], mixed[i]["c_tangent"]), i
+        assert alone["pcg_its"] == mixed[i]["pcg_its"]
+    assert normwise_rel(mixed[1]["c_tangent"], _oracle_tangent(wide)) <= TOL
+
+
 @pytest.mark.parametrize("use_graph", [True, False])
 def test_one_and_two_solver_streams_identical(use_graph):
     """Two stream groups (RVEs split in slab order, group 1 forked after group
