@@ -394,16 +394,21 @@ def main():
 
     B, n = args.batch, args.n
     specs = [configs.c2_spec(2, i, n=n) for i in range(B)]
+    # the batch engine's run length: as given, else the autotuner's, else the
+    # engine default (the serving leg keeps the engine default unless given)
+    batch_planes = args.march_planes
     if args.slab == 0 or args.inner_rtol == 0.0:
         from madmax_amd import autotune
         t = time.perf_counter()
         tuned = autotune.tune(specs[: args.tune_batch or B], rtol=args.rtol, log=log, use_cache=False)
-        log("autotune (%.1f s): slab_nodes=%d inner_rtol=%g" % (time.perf_counter() - t,
-                                                                tuned["slab_nodes"], tuned["inner_rtol"]))
+        log("autotune (%.1f s): slab_nodes=%d march_planes=%d inner_rtol=%g"
+            % (time.perf_counter() - t, tuned["slab_nodes"], tuned["march_planes"], tuned["inner_rtol"]))
         if args.slab == 0:
             args.slab = tuned["slab_nodes"]
         if args.inner_rtol == 0.0:
             args.inner_rtol = tuned["inner_rtol"]
+        if batch_planes < 0:
+            batch_planes = tuned["march_planes"]
     rows = 3 * (n + 1) ** 3
     per_rve = rows * 900 + (1 << 20)
     budget = int(per_rve * B * 1.3) + (256 << 20)
@@ -413,8 +418,8 @@ def main():
         cfg = dict(nrhs=6, precision=mm.MM_PREC_MIXED, hbm_budget=budget, max_rves=B + 8, slab_nodes=args.slab,
                    inner_rtol=args.inner_rtol, rtol=args.rtol, steps_per_launch=args.spl, use_graph=True,
                    ell_storage=mm.MM_ELL_DICT, solver_streams=args.streams)
-        if args.march_planes >= 0:
-            cfg["march_planes"] = args.march_planes
+        if batch_planes >= 0:
+            cfg["march_planes"] = batch_planes
         cfg.update(kw)
         return mm.Engine(**cfg)
 
@@ -624,7 +629,7 @@ def main():
         "data": "synthetic (seeded two-phase fiber RVEs)",
         "config": {"workload": "%d x %d^3 hex8 RVE tangents per step, 6-RHS block, "
                                "FP32 PCG + FP64 IR" % (B, n), "rve_elements": n ** 3,
-                   "batch": B, "slab_nodes": args.slab, "inner_rtol": args.inner_rtol,
+                   "batch": B, "slab_nodes": args.slab, "march_planes": batch_planes, "inner_rtol": args.inner_rtol,
                    "rtol": args.rtol, "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,

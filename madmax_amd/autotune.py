@@ -1,10 +1,13 @@
 """Autotuner of the batched solver: ELL slab shape (nodes per slab = lanes per
-workgroup) and the per-sweep FP32 PCG tolerance of the mixed-precision path.
+workgroup), the z-run length of the plane-march step kernel, and the
+per-sweep FP32 PCG tolerance of the mixed-precision path.
 
-The slab shape changes only how fast the kernels run, so each candidate is
-judged by the PCG step kernel's own duration on a resident sample batch
-(HIP events, mm_engine_time_step), which is stable where a whole small solve
-is not.  The sweep tolerance changes the iteration and sweep counts, so its
+The slab shape and the run length change only how fast the kernels run, so
+each candidate is judged by the PCG step kernel's own duration on a resident
+sample batch (HIP events, mm_engine_time_step), which is stable where a whole
+small solve is not.  Longer runs mean fewer, longer workgroups: faster for a
+large batch, slower to finish for a few RVEs at a time, so a serving engine
+keeps the engine default.  The sweep tolerance changes the iteration and sweep counts, so its
 candidates are judged by the best of two whole solves of the sample.  Every
 candidate must converge to the outer FP64 tolerance, so the choice only
 trades speed.  Results are cached as JSON keyed by device, RVE size, load
@@ -18,6 +21,8 @@ from . import _lib
 from .engine import Engine, unit_loads
 
 SLAB_NODES = (64, 128, 256)
+MARCH_PLANES = (8, 16)
+BASE_PLANES = 8
 INNER_RTOLS = (1e-3, 1e-4, 1e-5, 1e-6)
 BASE_INNER = 1e-5
 
@@ -29,9 +34,9 @@ def _cache_path():
     return os.path.join(base, "tune.json")
 
 
-def _solve_time(specs, slab_nodes, inner_rtol, rtol, nrhs, precision, budget):
+def _solve_time(specs, slab_nodes, inner_rtol, rtol, nrhs, precision, budget, march_planes=BASE_PLANES):
     with Engine(nrhs=nrhs, precision=precision, hbm_budget=budget, max_rves=len(specs) + 4,
-                slab_nodes=slab_nodes, inner_rtol=inner_rtol, rtol=rtol) as eng:
+                slab_nodes=slab_nodes, inner_rtol=inner_rtol, rtol=rtol, march_planes=march_planes) as eng:
         def run(keyed):
             for i, s in enumerate(specs):
                 eng.submit(i + 1, s, unit_loads(), _lib.MM_TANGENT_UNIT,
@@ -52,9 +57,10 @@ def _solve_time(specs, slab_nodes, inner_rtol, rtol, nrhs, precision, budget):
 
 def tune(specs, nrhs=6, precision=_lib.MM_PREC_MIXED, rtol=1e-9, budget=None, use_cache=True,
          log=None):
-    """Returns {"slab_nodes", "inner_rtol", "trials": [...]} for this workload."""
+    """Returns {"slab_nodes", "march_planes", "inner_rtol", "trials": [...]} for this
+    workload."""
     n = specs[0].nex
-    key = "%s|n%d|k%d|p%d|b%d|r%g" % (os.environ.get("HIP_VISIBLE_DEVICES", "0"), n, nrhs,
+    key = "%s|n%d|k%d|p%d|b%d|r%g|m" % (os.environ.get("HIP_VISIBLE_DEVICES", "0"), n, nrhs,
                                        precision, len(specs), rtol)
     path = _cache_path()
     cache = {}
@@ -72,7 +78,7 @@ def tune(specs, nrhs=6, precision=_lib.MM_PREC_MIXED, rtol=1e-9, budget=None, us
     best = None
     for sn in SLAB_NODES:
         r = _solve_time(specs, sn, BASE_INNER, rtol, nrhs, precision, budget)
-        r.update(slab_nodes=sn, inner_rtol=BASE_INNER)
+        r.update(slab_nodes=sn, march_planes=BASE_PLANES, inner_rtol=BASE_INNER)
         trials.append(r)
         if log:
             log("tune slab_nodes=%d: step kernel %.3f ms, solve %.3f s, %d its"
@@ -80,20 +86,33 @@ def tune(specs, nrhs=6, precision=_lib.MM_PREC_MIXED, rtol=1e-9, budget=None, us
         if r["ok"] and (best is None or r["step_ms"] < best["step_ms"]):
             best = r
     sn = best["slab_nodes"] if best else 128
+    if best is not None:
+        for mp in MARCH_PLANES:
+            if mp == BASE_PLANES:
+                continue
+            r = _solve_time(specs, sn, BASE_INNER, rtol, nrhs, precision, budget, march_planes=mp)
+            r.update(slab_nodes=sn, march_planes=mp, inner_rtol=BASE_INNER)
+            trials.append(r)
+            if log:
+                log("tune march_planes=%d: step kernel %.3f ms, solve %.3f s, %d its"
+                    % (mp, r["step_ms"], r["seconds"], r["its"]))
+            if r["ok"] and r["step_ms"] < best["step_ms"]:
+                best = r
+    mp = best["march_planes"] if best else BASE_PLANES
     # the tolerance sweep compares against a converged slab-shape trial; with
     # none, the defaults stand
     if precision == _lib.MM_PREC_MIXED and best is not None:
         for tol in INNER_RTOLS:
             if tol == BASE_INNER:
                 continue
-            r = _solve_time(specs, sn, tol, rtol, nrhs, precision, budget)
-            r.update(slab_nodes=sn, inner_rtol=tol)
+            r = _solve_time(specs, sn, tol, rtol, nrhs, precision, budget, march_planes=mp)
+            r.update(slab_nodes=sn, march_planes=mp, inner_rtol=tol)
             trials.append(r)
             if log:
                 log("tune inner_rtol=%g: %.3f s, %d its" % (tol, r["seconds"], r["its"]))
             if r["ok"] and r["seconds"] < best["seconds"]:
                 best = r
-    out = dict(slab_nodes=best["slab_nodes"] if best else 128,
+    out = dict(slab_nodes=best["slab_nodes"] if best else 128, march_planes=mp,
                inner_rtol=best["inner_rtol"] if best else 1e-5, trials=trials)
     cache[key] = out
     try:

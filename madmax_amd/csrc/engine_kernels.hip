@@ -1178,11 +1178,16 @@ __device__ void march_dict_fill(const EngPtrs &P, const RveDev *R, int slot, int
   const int tid = threadIdx.x;
   for (int i = tid; i < kDictPat; i += blockDim.x) slot_of[i] = 0;
   __syncthreads();
-  if (g.plane_int)
-    for (int z = max(g.z0, 1); z < min(g.z1, g.nz - 1); z++) {
+  // the pattern bytes of every plane are loaded before the first is used
+  // (one memory round trip, not one per plane)
+  if (g.plane_int) {
+    const int za = max(g.z0, 1), zb = min(g.z1, g.nz - 1);
+#pragma unroll 8
+    for (int z = za; z < zb; z++) {
       const int n = z * g.nxy + g.j;
       slot_of[P.npat[(long long)(fs + (n >> P.sn_shift)) * P.SN + (n & (P.SN - 1))]] = 1;  // same value from every writer
     }
+  }
   __syncthreads();
   if (tid < 64) {  // the first wavefront numbers the patterns in use
     int base = 0;
@@ -1201,10 +1206,27 @@ __device__ void march_dict_fill(const EngPtrs &P, const RveDev *R, int slot, int
     if (tid == 0) s_cnt = min(base, kMarchDictCap);
   }
   __syncthreads();
-  const TS *D0 = (const TS *)P.dict + (size_t)slot * kDictPat * kDictEntry;
-  for (int e = tid; e < s_cnt * kDictEntry; e += blockDim.x) {
-    const int sidx = e / kDictEntry, off = e - sidx * kDictEntry;
-    dv[sidx * kDictEntry + off] = D0[(size_t)s_pat[sidx] * kDictEntry + off];
+  // entries in 16-byte pieces (an entry is 81 or 162 of them), four loads in
+  // flight per lane before the first store
+  constexpr int kPieces = kDictEntry * (int)sizeof(TS) / 16;
+  const uint4 *D0 = reinterpret_cast<const uint4 *>((const TS *)P.dict + (size_t)slot * kDictPat * kDictEntry);
+  uint4 *dq = reinterpret_cast<uint4 *>(dv);
+  const int total = s_cnt * kPieces;
+  for (int e0 = tid; e0 < total; e0 += 4 * (int)blockDim.x) {
+    // pieces past the end load the last one again (no branch between the
+    // loads, so they stay in registers and in flight together)
+    uint4 t[4];
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      const int e = min(e0 + u * (int)blockDim.x, total - 1);
+      const int sidx = e / kPieces;
+      t[u] = D0[(size_t)s_pat[sidx] * kPieces + (e - sidx * kPieces)];
+    }
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      const int e = e0 + u * (int)blockDim.x;
+      if (e < total) dq[e] = t[u];
+    }
   }
   __syncthreads();
 }
@@ -1907,9 +1929,11 @@ static int launch_steps(const EngPtrs &P, const SolveParams &sp, int grid, int f
       case 0:
         // with the march on (a_lds > 0), its tiles and, if any RVE is
         // processed slab by slab (a_slab), k_step_a for those
-        if (P.a_lds > 0 && P.a_grid > 0) {
-          if (DICT) k_step_a_march_dict<TS, K><<<P.a_grid, P.SN, P.a_lds, s>>>(P, sp);
-          else k_step_a_march_full<TS, K><<<P.a_grid, P.SN, P.a_lds, s>>>(P, sp);
+        if constexpr (sizeof(TS) == 4) {  // the march is FP32 only
+          if (P.a_lds > 0 && P.a_grid > 0) {
+            if (DICT) k_step_a_march_dict<TS, K><<<P.a_grid, P.SN, P.a_lds, s>>>(P, sp);
+            else k_step_a_march_full<TS, K><<<P.a_grid, P.SN, P.a_lds, s>>>(P, sp);
+          }
         }
         if (P.a_lds == 0 || P.a_slab) k_step_a<TS, K, DICT><<<grid, P.SN, 0, s>>>(P, sp);
         break;
@@ -1937,7 +1961,8 @@ template <typename TS, int K, bool DICT>
 static int launch_time(const EngPtrs &P, int grid, int what, hipStream_t s)
 {
   if (what == 1) {
-    if (P.a_lds > 0) k_march_time<TS, K, DICT><<<grid, P.SN, P.a_lds, s>>>(P);
+    if constexpr (sizeof(TS) == 4)
+      if (P.a_lds > 0) k_march_time<TS, K, DICT><<<grid, P.SN, P.a_lds, s>>>(P);
     if (P.a_lds == 0 || P.a_slab) k_inner_time<TS, K, DICT><<<grid, P.SN, 0, s>>>(P);
   } else {
     k_spmv_time<TS, K, DICT><<<grid, P.SN, 0, s>>>(P);

@@ -307,6 +307,28 @@ def test_spmv_timing_runs(storage):
     assert ms2 > 0 and nbytes2 > nbytes
 
 
+def test_autotuner_picks_a_converging_configuration():
+    """The autotuner tries every slab shape, the march run lengths and the
+    sweep tolerances on a small sample; it returns candidates it measured,
+    all of which converged, and the chosen engine reproduces the tangents."""
+    from madmax_amd import autotune
+    specs = [configs.c2_spec(4, i, n=10) for i in range(3)]
+    out = autotune.tune(specs, budget=GB, use_cache=False)
+    assert out["slab_nodes"] in autotune.SLAB_NODES
+    assert out["march_planes"] in autotune.MARCH_PLANES
+    assert out["inner_rtol"] in autotune.INNER_RTOLS
+    tried = {(t["slab_nodes"], t["march_planes"], t["inner_rtol"]) for t in out["trials"]}
+    assert (out["slab_nodes"], out["march_planes"], out["inner_rtol"]) in tried
+    assert {t["slab_nodes"] for t in out["trials"]} == set(autotune.SLAB_NODES)
+    assert all(t["ok"] and t["step_ms"] > 0 for t in out["trials"])
+    with Engine(nrhs=6, hbm_budget=GB, max_rves=8, slab_nodes=out["slab_nodes"],
+                march_planes=out["march_planes"], inner_rtol=out["inner_rtol"], rtol=1e-9) as eng:
+        res = _solve_batch(eng, specs)
+    for s, r in zip(specs, res):
+        assert r["status"] == 0
+        assert normwise_rel(r["c_tangent"], _oracle_tangent(s)) <= TOL
+
+
 @pytest.mark.parametrize("precision", [MM_PREC_FP64, MM_PREC_MIXED])
 def test_single_load_sequential_reuse(precision):
     """One load case at a time on a resident RVE (nrhs 1, host check after
