@@ -53,6 +53,24 @@ def test_fe2_stress_c1(solver):
     assert st["pcg_its"] > 0
 
 
+@pytest.mark.parametrize("case", [0, 1, 2])
+def test_fp64_pcg_iterations_equal_oracle(case):
+    """The FP64 path (Jacobi-PCG to 1e-10 of the first residual, Newton to
+    nl_min_norm) takes exactly the oracle's PCG iterations on the same
+    system: its step, direction and stopping logic are the reference's, one
+    step per iteration (a step that updates nothing would add one per
+    Newton sweep)."""
+    spec, eps = [(configs.c1_spec(), configs.c1_strain()),
+                 (rve_spec(6, [(0.5, 0.5, 0.3)]), np.array([1e-3, 2e-4, -1e-4, 0.0, 3e-4, 0.0])),
+                 (rve_spec((7, 5, 4), [(0.4, 0.6, 0.25)]), np.array([0.0, 1e-3, 0.0, 5e-4, 0.0, -2e-4]))][case]
+    ref, _ = oracle_rve(spec)
+    out = ref.homog(eps, rtol=1e-10)
+    with Micro(spec, solver=MICRO_SOLVER_PCG) as m:
+        m.fe2(eps)
+        st = m.stats()
+    assert st["pcg_its"] == out["its"]
+
+
 def test_tangent_linear_c1_and_shortcut():
     spec = configs.c1_spec()
     ref, _ = oracle_rve(spec)
