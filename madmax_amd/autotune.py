@@ -25,6 +25,7 @@ MARCH_PLANES = (8, 16)
 BASE_PLANES = 8
 INNER_RTOLS = (1e-3, 1e-4, 1e-5, 1e-6)
 BASE_INNER = 1e-5
+NOISE = 0.97
 
 
 def _cache_path():
@@ -110,7 +111,9 @@ def tune(specs, nrhs=6, precision=_lib.MM_PREC_MIXED, rtol=1e-9, budget=None, us
             trials.append(r)
             if log:
                 log("tune inner_rtol=%g: %.3f s, %d its" % (tol, r["seconds"], r["its"]))
-            if r["ok"] and r["seconds"] < best["seconds"]:
+            # whole solves of a small sample differ by a percent from run to
+            # run: a tolerance must beat the current choice by more than that
+            if r["ok"] and r["seconds"] < NOISE * best["seconds"]:
                 best = r
     out = dict(slab_nodes=best["slab_nodes"] if best else 128, march_planes=mp,
                inner_rtol=best["inner_rtol"] if best else 1e-5, trials=trials)
