@@ -38,6 +38,9 @@ replicas and the streaming probes (the workload of the counter runs).
       PCG iterations, refinement sweeps, SpMV GB/s of both storages, and the
       mixed-precision stress against an FP64 solve and the oracle fixture
   c4  512 RVEs of uneven size (8^3..32^3), 6 load cases each: tangents/s
+  ell the reference ELL API's device mirror (one 30^3 and one 100^3 RVE):
+      FP64 SpMV, and FP64 Jacobi-PCG ms per iteration with four launches
+      per iteration and with the persistent kernel
   c5  FE^2 serving stream (12^3/20^3/30^3 RVEs, strain + tangent per request,
       Poisson arrivals): a sweep of arrival rates (throughput, p50 / p99,
       the saturation knee), then the full 32,768-request stream with a
@@ -106,11 +109,49 @@ def stream_ceilings(mm, nbytes=4 << 30, reps=10):
     return out
 
 
-def ell_api_spmv(mm, spec, reps=20):
+# Algorithmic bytes of one FP64 Jacobi-PCG iteration per row of the device
+# mirror: the matrix (81 slots x (8 B value + 4 B column id)), q = A p with p
+# formed from z and p_old (reads z, p_old; writes p, q), the update (reads x,
+# p, r, q, dinv; writes x, r, z).
+PCG_ROW_BYTES = 81 * 12 + 4 * 8 + 8 * 8
+
+
+def pcg_ms_per_iteration(mm, h, rows, fused, n1, n2):
+    """ms per FP64 Jacobi-PCG iteration of ell_device_solve_pcg on mirror h
+    with `fused` iterations per persistent launch (0: four launches per
+    iteration): the slope between solves capped at n1 and n2 iterations
+    (zero start, random right-hand side, a target no solve reaches), so the
+    transfers of b and x and the set-up cancel.  Best of two per cap."""
+    import ctypes as C
+    L = mm.lib()
+    mm._lib.check(L.ell_device_set_pcg_fused(h, fused), "ell_device_set_pcg_fused")
+    b = np.random.default_rng(3).standard_normal(rows)
+    bp = b.ctypes.data_as(C.POINTER(C.c_double))
+
+    def run(n):
+        x = np.zeros(rows)
+        s = mm._lib.EllSolver(n, 0, 1e-300, 0.0)
+        t = time.perf_counter()
+        rc = L.ell_device_solve_pcg(h, C.byref(s), bp, x.ctypes.data_as(C.POINTER(C.c_double)))
+        dt = time.perf_counter() - t
+        if rc != 5 or s.its != n:
+            raise RuntimeError("pcg timing solve: rc %d after %d of %d iterations" % (rc, s.its, n))
+        return dt
+
+    run(n1)
+    t1 = min(run(n1) for _ in range(2))
+    t2 = min(run(n2) for _ in range(2))
+    return (t2 - t1) / (n2 - n1) * 1e3
+
+
+def ell_api_spmv(mm, spec, reps=20, pcg_its=(60, 260)):
     """The reference ELL API's y = A x on the device mirror (FP64 values,
     int32 column ids, 81 slots per row, bit-exact slot order) for the FP64
     matrix of one RVE, assembled on the GPU by an FP64 engine (no solve).
-    Bytes per launch: N * 81 * (8 + 4) + 16 N (matrix, x once, y)."""
+    Bytes per launch: N * 81 * (8 + 4) + 16 N (matrix, x once, y).  Then the
+    FP64 Jacobi-PCG of ell_device_solve_pcg on the same mirror, ms per
+    iteration with four launches per iteration and with the persistent
+    kernel at 1, 32 and all iterations per launch."""
     import ctypes as C
     rows = 3 * (spec.nex + 1) * (spec.ney + 1) * (spec.nez + 1)
     with mm.Engine(nrhs=1, precision=mm.MM_PREC_FP64, hbm_budget=int(rows * 81 * 16 * 1.3) + (256 << 20),
@@ -126,14 +167,22 @@ def ell_api_spmv(mm, spec, reps=20):
     L = mm.lib()
     mm._lib.check(L.ell_device_create(C.byref(e.m), 0, C.byref(h)), "ell_device_create")
     ms = C.c_double()
+    pcg = None
     try:
         mm._lib.check(L.ell_device_time_mvp(h, reps, C.byref(ms)), "ell_device_time_mvp")
+        if pcg_its:
+            pcg = {"bytes_per_iteration": rows * PCG_ROW_BYTES}
+            for name, f in (("four_launches", 0), ("fused_1", 1), ("fused_32", 32), ("fused_all", 1 << 30)):
+                t = pcg_ms_per_iteration(mm, h, rows, f, *pcg_its)
+                pcg[name] = dict(ms_per_it=t, gbs=rows * PCG_ROW_BYTES / (t * 1e-3) / 1e9)
+                log("fp64 pcg (%s), %d rows: %.4f ms/iteration -> %.0f GB/s"
+                    % (name, rows, t, pcg[name]["gbs"]))
     finally:
         L.ell_device_destroy(h)
         e.free()
     nb = rows * 81 * 12 + 16 * rows
     gbs = nb / (ms.value * 1e-3) / 1e9
-    return dict(ms=ms.value, bytes=nb, gbs=gbs, frac=gbs / HBM_PEAK_GBS, rows=rows)
+    return dict(ms=ms.value, bytes=nb, gbs=gbs, frac=gbs / HBM_PEAK_GBS, rows=rows, pcg=pcg)
 
 
 def c3_fixture():
@@ -189,7 +238,7 @@ def workload_c3(args, mm, configs):
     out = {"workload": "c3: one 100^3 hex8 RVE (%d dofs), 4 fibers vf 0.4, strain ~ N(0, 1e-3)" % rows}
     res = c3_solves(mm, configs, ("mixed_dict", "mixed_full", "fp64_dict"))
     out.update(c3_report(res, c3_fixture()))
-    out["ell_api_spmv"] = ell_api_spmv(mm, configs.c3_spec())
+    out["ell_api_spmv"] = ell_api_spmv(mm, configs.c3_spec(), pcg_its=(20, 100))
     log("c3 reference-layout FP64 SpMV (ell_device_mvp): %.3f ms -> %.0f GB/s"
         % (out["ell_api_spmv"]["ms"], out["ell_api_spmv"]["gbs"]))
     a, b = res["mixed_dict"]["result"]["stress"][0], res["fp64_dict"]["result"]["stress"][0]
@@ -222,6 +271,16 @@ def workload_c4(args, mm, configs):
            "max_tangent_rel_err_vs_oracle": max(errs) if errs else None,
            "oracle_checked": len(errs)}
     log("c4: %.3f s -> %.1f tangents/s" % (dt, len(specs) / dt))
+    return out
+
+
+def workload_ell(args, mm, configs):
+    """The reference ELL API's device mirror on one 30^3 RVE of the batch and
+    on the 100^3 RVE: the bit-exact FP64 SpMV and the FP64 Jacobi-PCG per
+    iteration, four launches per iteration against the persistent kernel."""
+    out = {"workload": "ell: reference-layout FP64 ELL of one 30^3 and one 100^3 RVE, SpMV and Jacobi-PCG"}
+    out["rve_30"] = ell_api_spmv(mm, configs.c2_spec(2, 0, n=30))
+    out["rve_100"] = ell_api_spmv(mm, configs.c3_spec(), pcg_its=(20, 100))
     return out
 
 
@@ -374,7 +433,7 @@ def main():
     ap.add_argument("--march-planes", type=int, default=-1,
                     help="kernel A's plane-march run length (0: slab by slab; -1: engine default)")
     ap.add_argument("--kernels-only", action="store_true")
-    ap.add_argument("--workload", default="c2", choices=("c2", "c3", "c4", "c5"))
+    ap.add_argument("--workload", default="c2", choices=("c2", "c3", "c4", "c5", "ell"))
     ap.add_argument("--rate", type=float, default=0.0, help="c5 full-stream arrival rate (requests/s)")
     ap.add_argument("--rates", default="", help="c5 sweep rates, comma separated")
     ap.add_argument("--count", type=int, default=0, help="c5 requests per sweep point")
@@ -388,7 +447,8 @@ def main():
     from madmax_amd import configs
 
     if args.workload != "c2":
-        line = {"c3": workload_c3, "c4": workload_c4, "c5": workload_c5}[args.workload](args, mm, configs)
+        line = {"c3": workload_c3, "c4": workload_c4, "c5": workload_c5,
+                "ell": workload_ell}[args.workload](args, mm, configs)
         print(json.dumps(line), flush=True)
         return
 
@@ -517,7 +577,7 @@ def main():
         c3 = c3_report(res3, c3_fixture())
         c3["full_equals_dict"] = bool(np.array_equal(res3["mixed_full"]["result"]["stress"][0],
                                                      res3["mixed_dict"]["result"]["stress"][0]))
-        c3["ell_api_fp64"] = ell_api_spmv(mm, configs.c3_spec())
+        c3["ell_api_fp64"] = ell_api_spmv(mm, configs.c3_spec(), pcg_its=(20, 100))
         log("c3 reference-layout FP64 SpMV (ell_device_mvp): %.3f ms -> %.0f GB/s"
             % (c3["ell_api_fp64"]["ms"], c3["ell_api_fp64"]["gbs"]))
 

@@ -128,6 +128,7 @@ SIGNATURES = {
     "ell_device_solve_pcg": (C.c_int, [C.c_void_p, C.POINTER(EllSolver), dp, dp]),
     "ell_device_solve_jacobi": (C.c_int, [C.c_void_p, C.POINTER(EllSolver), dp, dp]),
     "ell_device_destroy": (C.c_int, [C.c_void_p]),
+    "ell_device_set_pcg_fused": (C.c_int, [C.c_void_p, C.c_int]),
     "ell_device_solve_pcg_noalloc": (C.c_int, [C.c_void_p, C.POINTER(EllSolver), C.c_void_p, C.c_void_p]),
     # include/micro_api.h
     "micro_config_default": (C.c_int, [C.POINTER(MicroConfig)]),

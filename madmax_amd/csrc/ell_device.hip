@@ -207,8 +207,64 @@ __global__ void k_pcg_init(const int *__restrict__ cols, const double *__restric
     for (int i = 0; i < 3; i++) part[blockIdx.x * 4 + i] = v[i];
 }
 
-// q = A p with p = z + beta p_old formed on the fly (both for the own rows,
-// which are stored, and for every gathered column); partial p.q (part[.][0])
+// Row `row` (lane `lane` of slab `slab`) of q = A p with p = z + beta p_old
+// formed on the fly, both for the own row, which is stored, and for every
+// gathered column; returns q_row * p_row.  Shared by the per-step kernels and
+// the fused kernel, so both do the same arithmetic.
+__device__ inline double pcg_q_row(const int *__restrict__ cols, const double *__restrict__ vals, int nnz, int S,
+                                   int slab, int lane, long long row, const double *__restrict__ z, double beta,
+                                   const double *__restrict__ pold, double *__restrict__ pnew,
+                                   double *__restrict__ q)
+{
+  const long long base = (long long)slab * S * nnz;
+  double acc = 0.0;
+  // kU slots per round trip, as in row_dot_exact; accumulated in slot order
+  for (int j0 = 0; j0 < nnz; j0 += kU) {
+    int c[kU];
+    double v[kU], t[kU];
+#pragma unroll
+    for (int u = 0; u < kU; u++) {
+      const bool in = j0 + u < nnz;
+      const long long e = base + (long long)(j0 + u) * S + lane;
+      c[u] = in ? __builtin_nontemporal_load(&cols[e]) : -1;
+      v[u] = in ? __builtin_nontemporal_load(&vals[e]) : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < kU; u++) t[u] = (c[u] >= 0) ? z[c[u]] + beta * pold[c[u]] : 0.0;
+    bool stop = false;
+#pragma unroll
+    for (int u = 0; u < kU; u++) {
+      if (c[u] == -1) {
+        stop = true;
+        break;
+      }
+      acc += v[u] * t[u];
+    }
+    if (stop) break;
+  }
+  const double pi = z[row] + beta * pold[row];
+  pnew[row] = pi;
+  q[row] = acc;
+  return pi * acc;
+}
+
+// Row `row` of x += alpha p, r -= alpha q, z = dinv r; v = {r.r, r.z} terms.
+__device__ inline void pcg_update_row(long long row, double alpha, const double *__restrict__ dinv,
+                                      const double *__restrict__ p, const double *__restrict__ q,
+                                      double *__restrict__ x, double *__restrict__ r, double *__restrict__ z,
+                                      double (&v)[2])
+{
+  const double xi = x[row] + alpha * p[row];
+  const double ri = r[row] - alpha * q[row];
+  const double zi = dinv[row] * ri;
+  x[row] = xi;
+  r[row] = ri;
+  z[row] = zi;
+  v[0] = ri * ri;
+  v[1] = ri * zi;
+}
+
+// q = A p (pcg_q_row); partial p.q (part[.][0])
 __global__ void k_pcg_spmv(const int *__restrict__ cols, const double *__restrict__ vals, int nrow,
                            int nnz, int S, const double *__restrict__ z, double *__restrict__ pA,
                            double *__restrict__ pB, double *__restrict__ q,
@@ -222,19 +278,7 @@ __global__ void k_pcg_spmv(const int *__restrict__ cols, const double *__restric
   const int lane = threadIdx.x;
   const long long row = (long long)blockIdx.x * S + lane;
   double v[1] = {0.0};
-  if (row < nrow) {
-    const long long base = (long long)blockIdx.x * S * nnz;
-    double acc = 0.0;
-    for (int j = 0; j < nnz; j++) {
-      const int c = cols[base + (long long)j * S + lane];
-      if (c == -1) break;
-      acc += vals[base + (long long)j * S + lane] * (z[c] + beta * pold[c]);
-    }
-    const double pi = z[row] + beta * pold[row];
-    pnew[row] = pi;
-    q[row] = acc;
-    v[0] = pi * acc;
-  }
+  if (row < nrow) v[0] = pcg_q_row(cols, vals, nnz, S, blockIdx.x, lane, row, z, beta, pold, pnew, q);
   block_sum<1>(v, lds);
   if (threadIdx.x == 0) {
     part[blockIdx.x * 4] = v[0];
@@ -256,22 +300,175 @@ __global__ void k_pcg_update(int nrow, int S, const double *__restrict__ dinv,
   const double *p = st->pbuf ? pB : pA;  // buffer written by k_pcg_spmv
   const long long row = (long long)blockIdx.x * S + threadIdx.x;
   double v[2] = {0.0, 0.0};
-  if (row < nrow) {
-    const double xi = x[row] + alpha * p[row];
-    const double ri = r[row] - alpha * q[row];
-    const double zi = dinv[row] * ri;
-    x[row] = xi;
-    r[row] = ri;
-    z[row] = zi;
-    v[0] = ri * ri;
-    v[1] = ri * zi;
-  }
+  if (row < nrow) pcg_update_row(row, alpha, dinv, p, q, x, r, z, v);
   block_sum<2>(v, lds);
   if (threadIdx.x == 0) {
     part[blockIdx.x * 4] = v[0];
     part[blockIdx.x * 4 + 1] = v[1];
     part[blockIdx.x * 4 + 2] = 0.0;
   }
+}
+
+// ---- the same iteration as one persistent kernel ----------------------------
+// k_pcg_fused runs up to `its` PCG iterations in one cooperative launch of G
+// co-resident workgroups of 256 lanes (one slab each at a time, slabs
+// blockIdx.x, +G, ...).  An iteration is q = A p for the own slabs, a grid
+// barrier, p.q summed by every workgroup from the slab partials, the vector
+// update of the own slabs, a second barrier and the r.r / r.z sums.  Every
+// workgroup sums the same partials in the same order as k_reduce (256 lanes,
+// slab-strided, then block_sum), so all of them hold the same scalars, take
+// the same branch at the convergence test and match the per-step kernels bit
+// for bit.  Slab partials: p.q in part[.][0], r.r and r.z in part[.][1..2];
+// each is rewritten only after a barrier that every reader of the previous
+// value has passed.
+constexpr int kFusedLanes = 256;
+// Grid barrier words, each on a 256-byte line of its own (index * kBarLine):
+// line 0 the release word, line 1 the count of groups arrived, line 2 + g
+// the count of arrivals of group g (kBarGroup consecutive workgroups).
+constexpr int kBarLine = 64;
+constexpr int kBarGroup = 16;
+constexpr int kBarMaxGroups = 64;
+constexpr int kBarMaxGrid = kBarGroup * kBarMaxGroups;
+constexpr size_t kBarBytes = (size_t)(2 + kBarMaxGroups) * kBarLine * sizeof(unsigned);
+// Bound on a barrier wait in polls (each polls an agent-scope atomic, then
+// sleeps): far above any legitimate wait (the slowest workgroup's phase),
+// and a wait that reaches it marks the launch as faulted instead of hanging.
+constexpr unsigned kBarSpinLimit = 1u << 22;
+
+// Barrier n (1, 2, ... within a launch; the host zeroes the words before
+// it).  After the workgroup barrier, lane 0 arrives at its group's counter
+// with a release RMW at agent scope (the L2s of the XCDs are not coherent
+// with each other, so this writes the workgroup's stores back); the last
+// arrival of the group acquires and arrives at the group count; the last
+// group's arrival acquires and publishes n in the release word, which every
+// lane 0 polls and then acquires, before the workgroup barrier hands the
+// ordering on to the other waves (HIP's own grid sync uses the same chain).
+// Two levels keep the same-address atomics to 16 per group counter and one
+// per group at the top, and the polls off the counters' lines.  Returns
+// false when the wait gave up; the first workgroup to give up leaves the
+// fault flag set (fault[0]) and the group count it saw against the count it
+// waited for (fault[1], fault[2]; the IterState words after pad0).
+__device__ inline bool grid_barrier(unsigned *bar, unsigned n, int *fault, int &ok_lds)
+{
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const unsigned G = gridDim.x, g = blockIdx.x / kBarGroup;
+    const unsigned ngroups = (G + kBarGroup - 1) / kBarGroup;
+    const unsigned gsize = (G - g * kBarGroup) < kBarGroup ? (G - g * kBarGroup) : kBarGroup;
+    unsigned old = __hip_atomic_fetch_add(&bar[(2 + g) * kBarLine], 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    if (old + 1 == n * gsize) {
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+      old = __hip_atomic_fetch_add(&bar[kBarLine], 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+      if (old + 1 == n * ngroups) {
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        __hip_atomic_store(&bar[0], n, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+      }
+    }
+    unsigned spins = 0;
+    int ok = 1;
+    while (__hip_atomic_load(&bar[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < n) {
+      if (++spins > kBarSpinLimit ||
+          ((spins & 63) == 0 && __hip_atomic_load(fault, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))) {
+        ok = 0;
+        break;
+      }
+      __builtin_amdgcn_s_sleep(2);
+    }
+    if (!ok) {
+      __hip_atomic_store(fault + 1, (int)__hip_atomic_load(&bar[kBarLine], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT),
+                         __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __hip_atomic_store(fault + 2, (int)(n * ngroups), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __hip_atomic_store(fault, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    ok_lds = ok;
+  }
+  __syncthreads();
+  return ok_lds != 0;
+}
+
+__global__ __launch_bounds__(kFusedLanes) void k_pcg_fused(
+    const int *__restrict__ cols, const double *__restrict__ vals, int nrow, int nnz, int nslab,
+    const double *__restrict__ dinv, double *__restrict__ x, double *__restrict__ r, double *__restrict__ z,
+    double *__restrict__ pA, double *__restrict__ pB, double *__restrict__ q, double *__restrict__ part,
+    IterState *st, unsigned *bar, int its)
+{
+  constexpr int S = kFusedLanes;
+  __shared__ double lds[48];
+  __shared__ double bc[2];
+  __shared__ int ok_lds;
+  if (st->done) return;
+  const unsigned G = gridDim.x;
+  const int lane = threadIdx.x;
+  double beta = st->beta, rz = st->rz, alpha = st->alpha, rnorm = st->rnorm;
+  const double bnorm = st->bnorm, rtol = st->rtol;
+  int pbuf = st->pbuf, n = st->its, done = 0;
+  const int max_its = st->max_its;
+  int *fault = &st->pad0;
+  unsigned arrivals = 0;
+  int k = 0;
+  for (; k < its && !done; k++) {
+    const double *pold = pbuf ? pB : pA;
+    double *pnew = pbuf ? pA : pB;
+    for (int slab = blockIdx.x; slab < nslab; slab += G) {
+      const long long row = (long long)slab * S + lane;
+      double v[1] = {0.0};
+      if (row < nrow) v[0] = pcg_q_row(cols, vals, nnz, S, slab, lane, row, z, beta, pold, pnew, q);
+      block_sum<1>(v, lds);
+      if (lane == 0) part[slab * 4] = v[0];
+    }
+    if (!grid_barrier(bar, ++arrivals, fault, ok_lds)) return;
+    double s1[1], s[2];
+    group_sum_rows<1>(part, nslab, 4, s1, lds);
+    if (lane == 0) bc[0] = s1[0];
+    __syncthreads();
+    alpha = rz / bc[0];
+    pbuf ^= 1;
+    const double *p = pbuf ? pB : pA;  // = pnew
+    for (int slab = blockIdx.x; slab < nslab; slab += G) {
+      const long long row = (long long)slab * S + lane;
+      double v[2] = {0.0, 0.0};
+      if (row < nrow) pcg_update_row(row, alpha, dinv, p, q, x, r, z, v);
+      block_sum<2>(v, lds);
+      if (lane == 0) {
+        part[slab * 4 + 1] = v[0];
+        part[slab * 4 + 2] = v[1];
+      }
+    }
+    if (!grid_barrier(bar, ++arrivals, fault, ok_lds)) return;
+    group_sum_rows<2>(part + 1, nslab, 4, s, lds);
+    if (lane == 0) {
+      bc[0] = s[0];
+      bc[1] = s[1];
+    }
+    __syncthreads();
+    n += 1;
+    rnorm = sqrt(bc[0]);
+    if (rnorm <= rtol * bnorm || n >= max_its) {
+      done = 1;
+    } else {
+      beta = bc[1] / rz;
+      rz = bc[1];
+    }
+    __syncthreads();  // bc is rewritten by the next iteration
+  }
+  // Every workgroup read the state before its first arrival, and workgroup 0
+  // writes it only after that barrier completed.
+  if (blockIdx.x == 0 && lane == 0 && k > 0) {
+    st->its = n;
+    st->rnorm = rnorm;
+    st->rz = rz;
+    st->alpha = alpha;
+    st->beta = beta;
+    st->pbuf = pbuf;
+    st->done = done;
+  }
+}
+
+// Zeroes the barrier words of k_pcg_fused (a kernel, not a memset node).
+__global__ void k_bar_reset(unsigned *bar)
+{
+  if (threadIdx.x < 2 + kBarMaxGroups) bar[threadIdx.x * kBarLine] = 0u;
 }
 
 // x = 0 (a kernel, not a memset node, so that the solve can sit in a graph)
@@ -285,6 +482,7 @@ __global__ void k_zero(double *__restrict__ x, int n)
 struct PcgVecs {
   double *r, *z, *pA, *pB, *q, *part;
   IterState *st;
+  unsigned *bar;  // grid-barrier counter of k_pcg_fused
 };
 
 }  // namespace
@@ -296,6 +494,9 @@ struct ell_device_ {
   double *vals, *dinv;
   double *x, *b, *r, *z, *pA, *pB, *q, *part;
   IterState *st;
+  unsigned *bar;
+  int fused_its;   // 0: four launches per PCG iteration; n > 0: k_pcg_fused, n iterations per launch
+  int fused_grid;  // co-resident workgroups of k_pcg_fused (set with fused_its)
   hipStream_t stream;
 };
 
@@ -399,6 +600,7 @@ extern "C" int ell_device_create(const ell_matrix *m, int slab_rows, ell_device 
   ALLOC(d->q, nv * sizeof(double));
   ALLOC(d->part, (size_t)(d->nslab + 1) * 4 * sizeof(double));
   ALLOC(d->st, sizeof(IterState));
+  ALLOC(d->bar, kBarBytes);
 #undef ALLOC
   if (hipMemsetAsync(d->st, 0, sizeof(IterState), d->stream) != hipSuccess) {
     rc = MM_ERR_HIP;
@@ -547,6 +749,27 @@ static int pcg_run(ell_device *d, const PcgVecs &V, ell_solver *solver, const do
     k_zero<<<(d->nrow + 255) / 256, 256, 0, d->stream>>>(x, d->nrow);
     MM_HIP(hipGetLastError());
   }
+  while (!h.done && d->fused_its > 0) {
+    k_bar_reset<<<1, 128, 0, d->stream>>>(V.bar);
+    MM_HIP(hipGetLastError());
+    const int *cols = d->cols;
+    const double *vals = d->vals, *dinv = d->dinv;
+    int nrow = d->nrow, nnz = d->nnz, nslab = d->nslab, its = d->fused_its;
+    double *xv = x, *r = V.r, *z = V.z, *pA = V.pA, *pB = V.pB, *q = V.q, *part = V.part;
+    IterState *st = V.st;
+    unsigned *bar = V.bar;
+    void *args[] = {&cols, &vals, &nrow, &nnz, &nslab, &dinv, &xv, &r, &z, &pA, &pB, &q, &part, &st, &bar, &its};
+    MM_HIP(hipLaunchCooperativeKernel((const void *)k_pcg_fused, dim3(d->fused_grid), dim3(kFusedLanes), args, 0,
+                                      d->stream));
+    rc = get_state(d, V.st, &h);
+    if (rc) return rc;
+    if (h.pad0) {
+      fprintf(stderr,
+              "madmax_amd: k_pcg_fused: a grid barrier wait gave up at %d of %d group arrivals (%d workgroups)\n",
+              h.pad1, h.pad2, d->fused_grid);
+      return MM_ERR_HIP;
+    }
+  }
   while (!h.done) {
     for (int k = 0; k < 32; k++) {
       k_pcg_spmv<<<d->nslab, d->S, 0, d->stream>>>(d->cols, d->vals, d->nrow, d->nnz, d->S, V.z, V.pA, V.pB,
@@ -569,6 +792,31 @@ static int pcg_run(ell_device *d, const PcgVecs &V, ell_solver *solver, const do
   return (h.bnorm == 0.0 || h.rnorm <= rtol * h.bnorm) ? 0 : MM_NOT_CONVERGED;
 }
 
+extern "C" int ell_device_set_pcg_fused(ell_device *d, int its_per_launch)
+{
+  if (!d || its_per_launch < 0) return 1;
+  if (its_per_launch == 0) {
+    d->fused_its = 0;
+    return 0;
+  }
+  if (d->S != kFusedLanes || d->nslab == 0) return 1;
+  int coop = 0, ncu = 0, per_cu = 0;
+  MM_HIP(hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, d->device));
+  if (!coop) return 1;
+  MM_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, d->device));
+  MM_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)k_pcg_fused, kFusedLanes, 0));
+  if (ncu < 1 || per_cu < 1) return 1;
+  // Half of what fits: a grid at the full occupancy limit needs every wave
+  // slot of every CU free at once, and the launch admits it even when some
+  // are not (its barrier waits then run out).  Workgroups loop over slabs,
+  // so a smaller grid only lengthens each one's share.
+  long long cap = (long long)ncu * (per_cu > 1 ? per_cu / 2 : 1);
+  if (cap > kBarMaxGrid) cap = kBarMaxGrid;
+  d->fused_grid = (int)(d->nslab < cap ? d->nslab : cap);
+  d->fused_its = its_per_launch;
+  return 0;
+}
+
 extern "C" int ell_device_solve_pcg(ell_device *d, ell_solver *solver, const double *b, double *x)
 {
   if (!d || !b || !x) return 1;
@@ -576,7 +824,7 @@ extern "C" int ell_device_solve_pcg(ell_device *d, ell_solver *solver, const dou
   if (d->nrow == 0) return 0;
   MM_HIP(hipMemcpyAsync(d->x, x, d->nrow * sizeof(double), hipMemcpyHostToDevice, d->stream));
   MM_HIP(hipMemcpyAsync(d->b, b, d->nrow * sizeof(double), hipMemcpyHostToDevice, d->stream));
-  const PcgVecs V{d->r, d->z, d->pA, d->pB, d->q, d->part, d->st};
+  const PcgVecs V{d->r, d->z, d->pA, d->pB, d->q, d->part, d->st, d->bar};
   const int rc = pcg_run(d, V, solver, d->b, d->x);
   if (rc != 0 && rc != MM_NOT_CONVERGED) return rc;
   MM_HIP(hipMemcpyAsync(x, d->x, d->nrow * sizeof(double), hipMemcpyDeviceToHost, d->stream));
@@ -623,6 +871,7 @@ int mm::ell_device_alloc(int nrow, int nnz, int S, ell_device **out)
   ALLOC(d->q, nv * sizeof(double));
   ALLOC(d->part, (size_t)(d->nslab + 1) * 4 * sizeof(double));
   ALLOC(d->st, sizeof(IterState));
+  ALLOC(d->bar, kBarBytes);
 #undef ALLOC
   if (hipMemsetAsync(d->cols, 0xff, slots * sizeof(int), d->stream) != hipSuccess ||
       hipMemsetAsync(d->vals, 0, slots * sizeof(double), d->stream) != hipSuccess ||
@@ -651,7 +900,7 @@ int mm::ell_device_pcg_device(ell_device *d, ell_solver *solver, const double *b
 {
   if (!d || !b || !x) return MM_ERR_ARG;
   if (d->nrow == 0) return 0;
-  const PcgVecs V{d->r, d->z, d->pA, d->pB, d->q, d->part, d->st};
+  const PcgVecs V{d->r, d->z, d->pA, d->pB, d->q, d->part, d->st, d->bar};
   return pcg_run(d, V, solver, b, x);
 }
 
@@ -659,14 +908,15 @@ int mm::ell_device_pcg_device(ell_device *d, ell_solver *solver, const double *b
 // The PCG work vectors, partial sums and iteration state of
 // ell_device_solve_pcg_noalloc live in this file-scope array: five vectors of
 // ELL_NOALLOC_MAX_ROWS doubles, the partials of up to ELL_NOALLOC_MAX_ROWS / 64
-// slabs and one IterState (about 40.5 MiB, held on every GPU the library runs
-// on until the process ends).  Host code never uses the array's name as an
+// slabs, one IterState and the fused kernel's barrier counter (about 40.5 MiB,
+// held on every GPU the library runs on until the process ends).  Host code never uses the array's name as an
 // address: the per-device address comes from hipGetSymbolAddress.
 namespace {
 constexpr size_t kNaRows = ELL_NOALLOC_MAX_ROWS;
 constexpr size_t kNaPart = (kNaRows / 64 + 1) * 4;
 constexpr size_t kNaState = (sizeof(IterState) + 7) / 8;
-constexpr size_t kNaDoubles = 5 * kNaRows + kNaPart + kNaState;
+constexpr size_t kNaBar = kBarBytes / 8;  // the grid-barrier words of k_pcg_fused
+constexpr size_t kNaDoubles = 5 * kNaRows + kNaPart + kNaState + kNaBar;
 constexpr int kNaMaxDevices = 64;
 }  // namespace
 
@@ -712,7 +962,8 @@ extern "C" int ell_device_solve_pcg_noalloc(ell_device *d, ell_solver *solver, c
   const int rc = noalloc_scratch(d->device, &sp);
   if (rc) return rc;
   const PcgVecs V{sp, sp + kNaRows, sp + 2 * kNaRows, sp + 3 * kNaRows, sp + 4 * kNaRows, sp + 5 * kNaRows,
-                  reinterpret_cast<IterState *>(sp + 5 * kNaRows + kNaPart)};
+                  reinterpret_cast<IterState *>(sp + 5 * kNaRows + kNaPart),
+                  reinterpret_cast<unsigned *>(sp + 5 * kNaRows + kNaPart + kNaState)};
   return pcg_run(d, V, solver, b, x);
 }
 
@@ -732,6 +983,7 @@ extern "C" int ell_device_destroy(ell_device *d)
   (void)hipFree(d->q);
   (void)hipFree(d->part);
   (void)hipFree(d->st);
+  (void)hipFree(d->bar);
   if (d->stream) (void)hipStreamDestroy(d->stream);
   free(d);
   return 0;

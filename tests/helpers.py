@@ -1,5 +1,7 @@
 """Shared helpers of the test-suite: oracle RVEs built from the same specs
 the product consumes, and the error measures used in the parity bars."""
+import ctypes as C
+
 import numpy as np
 
 from madmax_amd import spec_fibers
@@ -27,3 +29,29 @@ def same_bits(a, b):
     a = np.ascontiguousarray(a, dtype=np.float64)
     b = np.ascontiguousarray(b, dtype=np.float64)
     return a.shape == b.shape and np.array_equal(a.view(np.uint64), b.view(np.uint64))
+
+
+class DevBuf:
+    """Device memory of the caller, from the HIP runtime the library itself
+    links (libamdhip64.so.7), so that no second runtime enters the process."""
+
+    def __init__(self, host):
+        self.hip = C.CDLL("libamdhip64.so.7")
+        self.n = len(host)
+        self.ptr = C.c_void_p()
+        assert self.hip.hipMalloc(C.byref(self.ptr), C.c_size_t(8 * self.n)) == 0
+        self.put(host)
+
+    def put(self, host):
+        h = np.ascontiguousarray(host, dtype=np.float64)
+        assert self.hip.hipMemcpy(self.ptr, h.ctypes.data_as(C.c_void_p), C.c_size_t(8 * self.n), 1) == 0
+
+    def get(self):
+        h = np.zeros(self.n)
+        assert self.hip.hipMemcpy(h.ctypes.data_as(C.c_void_p), self.ptr, C.c_size_t(8 * self.n), 2) == 0
+        return h
+
+    def free(self):
+        if self.ptr:
+            self.hip.hipFree(self.ptr)
+            self.ptr = C.c_void_p()

@@ -11,37 +11,11 @@ import pytest
 
 from madmax_amd import Ell, lib, rve_spec
 from madmax_amd._lib import EllSolver
-from tests.helpers import oracle_rve, same_bits
+from tests.helpers import DevBuf, oracle_rve, same_bits
 
 pytestmark = pytest.mark.gpu
 
 MAX_ROWS = 1 << 20
-
-
-class DevBuf:
-    """Device memory of the caller, from the HIP runtime the library itself
-    links (libamdhip64.so.7), so that no second runtime enters the process."""
-
-    def __init__(self, host):
-        self.hip = C.CDLL("libamdhip64.so.7")
-        self.n = len(host)
-        self.ptr = C.c_void_p()
-        assert self.hip.hipMalloc(C.byref(self.ptr), C.c_size_t(8 * self.n)) == 0
-        self.put(host)
-
-    def put(self, host):
-        h = np.ascontiguousarray(host, dtype=np.float64)
-        assert self.hip.hipMemcpy(self.ptr, h.ctypes.data_as(C.c_void_p), C.c_size_t(8 * self.n), 1) == 0
-
-    def get(self):
-        h = np.zeros(self.n)
-        assert self.hip.hipMemcpy(h.ctypes.data_as(C.c_void_p), self.ptr, C.c_size_t(8 * self.n), 2) == 0
-        return h
-
-    def free(self):
-        if self.ptr:
-            self.hip.hipFree(self.ptr)
-            self.ptr = C.c_void_p()
 
 
 def _rve_system(n, eps):
