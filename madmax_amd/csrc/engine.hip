@@ -772,6 +772,7 @@ static void fill_state(const mm_engine *e, const HostSlot &h, RveDev *st, int wa
   st->mnzc = mt.mnzc;
   st->mnt = mt.mnt;
   st->mncg = mt.mncg;
+  st->big = h.nslab >= kBigSlabs ? 1 : 0;
   for (int k = 0; k < h.req.nload; k++)
     for (int i = 0; i < MM_NVOI; i++) st->load[k][i] = h.req.strain[k][i];
   if (warm)

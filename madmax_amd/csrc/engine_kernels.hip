@@ -1046,7 +1046,11 @@ __global__ __launch_bounds__(256) void k_step_a(EngPtrs P, SolveParams sp)
     if (tid == 0) start_sweep<K>(N, s4, R, sp);
   } else if (phase0 == PH_INNER) {
     double s2[2 * K];
-    group_sum_cols<2 * K>(p0, nslab, P.part_stride, K, s2, lds);
+    if (R->big) {
+      for (int i = 0; i < 2 * K; i++) s2[i] = R->rr_rz[i];  // summed by kernel R
+    } else {
+      group_sum_cols<2 * K>(p0, nslab, P.part_stride, K, s2, lds);
+    }
     if (tid == 0) s_xupd = inner_check<K>(N, s2, sp) ? 1 : 0;
   }
   __syncthreads();
@@ -1544,7 +1548,11 @@ __device__ __forceinline__ void step_a_march(const EngPtrs &P, const SolveParams
     // scratch: the dynamic LDS, which the march has not started to use
     double(&plds)[16 * 2 * K] = *reinterpret_cast<double(*)[16 * 2 * K]>(march_dyn);
     double s2[2 * K];
-    group_sum_cols<2 * K>(p0, nslab, P.part_stride, K, s2, plds);
+    if (R->big) {
+      for (int i = 0; i < 2 * K; i++) s2[i] = R->rr_rz[i];  // summed by kernel R
+    } else {
+      group_sum_cols<2 * K>(p0, nslab, P.part_stride, K, s2, plds);
+    }
     if (tid == 0) s_xupd = inner_check<K>(N, s2, sp) ? 1 : 0;
   }
   __syncthreads();
@@ -1860,6 +1868,14 @@ __global__ __launch_bounds__(256) void k_step_r(EngPtrs P, SolveParams sp)
   }
   if (phase == PH_RES) res_slab<TS, K>(P, slab, slot, R, (R->warm || R->affine) && S->sweep == 0, lds);
   else if (phase == PH_AVG) avg_slab<TS, K>(P, slab, slot, R, lds);
+  else if (phase == PH_INNER && R->big && slab == R->first_slab) {
+    // B's r.r / r.z totals for kernel A (group_sum_cols over SN lanes, as A
+    // would form them)
+    double s2[2 * K];
+    group_sum_cols<2 * K>(slab_part(P, R->first_slab).p, R->nslab, P.part_stride, K, s2, lds);
+    if (tid == 0)
+      for (int i = 0; i < 2 * K; i++) R->rr_rz[i] = s2[i];
+  }
 }
 
 // Stand-alone copy of the INNER-phase SpMV (w = A z) over every resident RVE,

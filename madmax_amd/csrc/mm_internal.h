@@ -118,6 +118,12 @@ struct RveDev {
   // mnt * mncg <= nslab, is processed by the workgroup of slab
   // first_slab + t.  mcpp == 0: kernel A works slab by slab.
   int mcpp, mL, mnzc, mnt, mncg;
+  // big != 0 (nslab >= kBigSlabs): during PCG sweeps, kernel R sums B's r.r /
+  // r.z partials once into rr_rz (in its first slab's workgroup, in the
+  // order every workgroup of kernel A would use), and kernel A reads the
+  // totals instead of each workgroup re-summing nslab partials.
+  int big;
+  double rr_rz[2 * MM_MAX_RHS];
   double fx[MM_MAX_FIBERS], fy[MM_MAX_FIBERS], fr[MM_MAX_FIBERS];
   double young[kNmat], poisson[kNmat];
   double load[MM_MAX_RHS][MM_NVOI];
@@ -187,6 +193,10 @@ constexpr int kDictEntry = 27 * kDictBlk;
 // march of a 30^3 RVE (window 318 nodes) with its dictionary cache under
 // 40 KB, four workgroups per CU; three cases per round fit only three.
 constexpr int march_cases(int) { return 2; }
+// RVEs of at least this many slabs get their r.r / r.z totals from kernel R
+// (RveDev::big): below it the re-summing in kernel A costs less than the
+// extra latency of the single workgroup in R.
+constexpr int kBigSlabs = 1024;
 // Dictionary entries a march tile keeps in LDS, and the bytes of that cache
 // (entries, then one slot byte per pattern), a multiple of 16.
 constexpr int kMarchDictCap = 10;
