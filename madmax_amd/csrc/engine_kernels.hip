@@ -66,6 +66,8 @@ struct MarchLds {
 // so each XCD walks one contiguous run of slabs and the z rows that
 // neighbouring slabs of an RVE share stay in that XCD's L2.
 constexpr int kXcd = 8;
+// Largest grid of kernel R (a multiple of 8; k_step_r loops over the slabs).
+constexpr int kRGrid = 2048;
 __device__ __forceinline__ int xcd_slab()
 {
   const int per = gridDim.x / kXcd;
@@ -1848,15 +1850,12 @@ __device__ void avg_slab(const EngPtrs &P, int slab, int slot, const RveDev *R,
   }
 }
 
+// Kernel R's work on one slab of an RVE being solved.
 template <typename TS, int K>
-__global__ __launch_bounds__(256) void k_step_r(EngPtrs P, SolveParams sp)
+__device__ void r_slab(const EngPtrs &P, int slab, double (&lds)[RedLds<K>::size])
 {
-  __shared__ double lds[RedLds<K>::size];
-  int slab;
-  if (!active_slab(P, slab)) return;
   const int tid = threadIdx.x;
   const int slot = P.slab_rve[slab];
-  if (slot < 0) return;
   RveDev *R = P.rves + slot;
   const SolveState *S = &R->st[1];
   const int phase = S->phase;
@@ -1875,6 +1874,47 @@ __global__ __launch_bounds__(256) void k_step_r(EngPtrs P, SolveParams sp)
     group_sum_cols<2 * K>(slab_part(P, R->first_slab).p, R->nslab, P.part_stride, K, s2, lds);
     if (tid == 0)
       for (int i = 0; i < 2 * K; i++) R->rr_rz[i] = s2[i];
+  }
+}
+
+// Kernel R runs on a grid of at most kRGrid workgroups (launch_steps): in a
+// PCG sweep it only copies each RVE's state (first slab) and, for big RVEs,
+// sums r.r / r.z, so a workgroup per slab would mostly start and exit.
+// Workgroup b takes the positions j = b / 8 + m * (grid / 8) of its XCD's
+// part (b % 8) of the active list (the split of active_in).  Per round its
+// lanes look up one position each and collect the slabs with work (an RVE's
+// first slab, the residual and averaging phases) in LDS; the workgroup then
+// processes those, in any order (each writes only its own slab's outputs).
+template <typename TS, int K>
+__global__ __launch_bounds__(256) void k_step_r(EngPtrs P, SolveParams sp)
+{
+  __shared__ double lds[RedLds<K>::size];
+  __shared__ int work[256];
+  __shared__ int nwork;
+  const int *list = P.act_list;
+  const int n = list[0];
+  const int per = (n + kXcd - 1) / kXcd;
+  const int g8 = gridDim.x / kXcd, part = blockIdx.x % kXcd, j0 = blockIdx.x / kXcd;
+  const int mine = (j0 < per) ? (per - 1 - j0) / g8 + 1 : 0;
+  const int tid = threadIdx.x;
+  for (int m0 = 0; m0 < mine; m0 += blockDim.x) {
+    if (tid == 0) nwork = 0;
+    __syncthreads();
+    const int m = m0 + tid;
+    const int i = part * per + j0 + m * g8;
+    if (m < mine && i < n) {
+      const int slab = list[1 + i];
+      const int slot = P.slab_rve[slab];
+      if (slot >= 0) {
+        const RveDev *R = P.rves + slot;
+        const int ph = R->st[1].phase;
+        if (slab == R->first_slab || ph == PH_RES || ph == PH_AVG) work[atomicAdd(&nwork, 1)] = slab;
+      }
+    }
+    __syncthreads();
+    const int cnt = nwork;
+    for (int w = 0; w < cnt; w++) r_slab<TS, K>(P, work[w], lds);
+    __syncthreads();  // work and nwork are rewritten by the next round
   }
 }
 
@@ -1954,7 +1994,7 @@ static int launch_steps(const EngPtrs &P, const SolveParams &sp, int grid, int f
         if (P.a_lds == 0 || P.a_slab) k_step_a<TS, K, DICT><<<grid, P.SN, 0, s>>>(P, sp);
         break;
       case 1: k_step_b<TS, K><<<grid, P.SN, 0, s>>>(P, sp); break;
-      default: k_step_r<TS, K><<<grid, P.SN, 0, s>>>(P, sp); break;
+      default: k_step_r<TS, K><<<grid < kRGrid ? grid : kRGrid, P.SN, 0, s>>>(P, sp); break;
     }
   }
   MM_HIP(hipGetLastError());
