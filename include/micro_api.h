@@ -210,6 +210,12 @@ typedef struct {
                             whose x-extent exceeds slab_nodes/2 - 1 elements
                             stay slab by slab).  Same SpMV bits; the p.q
                             partial sums are grouped differently. */
+  int big_slabs;         /* RVEs of at least this many slabs (default 1024)
+                            have their per-iteration dot products summed once
+                            per RVE (kernels R and S) instead of in every
+                            workgroup of the kernel that uses them: the same
+                            bits, with work linear instead of quadratic in
+                            the RVE's slab count */
 } mm_engine_config;
 
 typedef struct {

@@ -80,7 +80,7 @@ class EngineConfig(C.Structure):
                 ("sync_every_step", C.c_int), ("ell_storage", C.c_int),
                 ("initial_guess", C.c_int), ("solver_streams", C.c_int),
                 ("profile_step_kernel", C.c_int),
-                ("march_planes", C.c_int)]
+                ("march_planes", C.c_int), ("big_slabs", C.c_int)]
 
 
 class Request(C.Structure):

@@ -211,15 +211,16 @@ struct mm_engine {
   // (march_planes > 0) kernel A runs over the tiles of the RVEs that march,
   // listed by their slabs at act_list + (2 + g) * (nslab_cap + 1), with
   // a_lds[g] bytes of dynamic LDS; a_slab[g]: the group also holds RVEs
-  // that go slab by slab.
+  // that go slab by slab; s_big[g]: it holds an RVE of kBigSlabs+ slabs
+  // (kernel S runs after A).
   int *act_list = nullptr, *st_act = nullptr;
   bool act_dirty = true;
   long long act_n[2] = {0, 0}, tile_n[2] = {0, 0};
-  int a_lds[2] = {0, 0}, a_slab[2] = {0, 0};
+  int a_lds[2] = {0, 0}, a_slab[2] = {0, 0}, s_big[2] = {0, 0};
   // one captured graph per set of launch shapes (grids: powers of two >= 64
   // covering each group's slabs and tiles, 0 for an empty group; kernel A's
   // LDS); the kernels read the active count from their list
-  std::map<std::array<int, 8>, std::pair<hipGraph_t, hipGraphExec_t>> graphs;
+  std::map<std::array<int, 10>, std::pair<hipGraph_t, hipGraphExec_t>> graphs;
   // stats / threading
   std::mutex step_mu;
   long long completed = 0, nsteps = 0;
@@ -261,6 +262,7 @@ struct mm_engine {
     P.a_grid = 0;
     P.a_lds = 0;
     P.a_slab = 0;
+    P.s_big = 0;
     P.dict = dict;
     P.dict64 = dict64;
     P.SN = SN;
@@ -343,6 +345,7 @@ extern "C" int mm_engine_config_default(mm_engine_config *c)
   c->solver_streams = 2;
   c->profile_step_kernel = 0;
   c->march_planes = 8;
+  c->big_slabs = kBigSlabs;
   return 0;
 }
 
@@ -360,6 +363,7 @@ extern "C" int mm_engine_create(const mm_engine_config *cfg, mm_engine **out)
   if (cfg->initial_guess != MM_GUESS_ZERO && cfg->initial_guess != MM_GUESS_AFFINE) return MM_ERR_ARG;
   if (cfg->solver_streams != 1 && cfg->solver_streams != 2) return MM_ERR_ARG;
   if (cfg->march_planes < 0 || cfg->march_planes > 4096) return MM_ERR_ARG;
+  if (cfg->big_slabs < 1) return MM_ERR_ARG;
   if (!(cfg->rtol >= 0) || !(cfg->inner_rtol > 0) || cfg->max_inner_its < 1 || cfg->max_sweeps < 1)
     return MM_ERR_ARG;
   int ndev = 0;
@@ -772,7 +776,7 @@ static void fill_state(const mm_engine *e, const HostSlot &h, RveDev *st, int wa
   st->mnzc = mt.mnzc;
   st->mnt = mt.mnt;
   st->mncg = mt.mncg;
-  st->big = h.nslab >= kBigSlabs ? 1 : 0;
+  st->big = h.nslab >= e->cfg.big_slabs ? 1 : 0;
   for (int k = 0; k < h.req.nload; k++)
     for (int i = 0; i < MM_NVOI; i++) st->load[k][i] = h.req.strain[k][i];
   if (warm)
@@ -948,12 +952,13 @@ static int update_active(mm_engine *e)
   const int ngroup = e->stream2 ? 2 : 1;
   const bool march = e->cfg.march_planes > 0 && !e->fp64;
   long long n[2] = {0, 0}, nt[2] = {0, 0};
-  int lds[2] = {0, 0}, slab_mode[2] = {0, 0};
+  int lds[2] = {0, 0}, slab_mode[2] = {0, 0}, big[2] = {0, 0};
   int g = 0;
   for (auto &r : runs) {
     const HostSlot &h = e->slots[r.second];
     if (g == 0 && ngroup == 2 && n[0] > 0 && 2 * n[0] >= total) g = 1;
     for (long long b = 0; b < h.nslab; b++) e->st_act[g * stride + 1 + n[g]++] = (int)(r.first + b);
+    if (h.nslab >= e->cfg.big_slabs) big[g] = 1;
     const MarchTiling m = march_tiling(e, h.req.rve, h.nslab);
     if (march && m.mcpp > 0) {
       for (long long t = 0; t < m.tiles(); t++) e->st_act[(2 + g) * stride + 1 + nt[g]++] = (int)(r.first + t);
@@ -976,6 +981,7 @@ static int update_active(mm_engine *e)
     e->tile_n[q] = nt[q];
     e->a_lds[q] = (march && nt[q] > 0) ? march_lds_class(lds[q]) : 0;
     e->a_slab[q] = slab_mode[q];
+    e->s_big[q] = big[q];
   }
   e->act_dirty = false;
   return 0;
@@ -1006,6 +1012,8 @@ static int queue_steps(mm_engine *e, int grid0, int grid1)
   P1.a_lds = e->a_lds[1];
   P0.a_slab = e->a_slab[0];
   P1.a_slab = e->a_slab[1];
+  P0.s_big = e->s_big[0];
+  P1.s_big = e->s_big[1];
   const SolveParams sp = e->params();
   const int nk = 3 * e->cfg.steps_per_launch;
   const bool f64 = e->fp64, dm = e->dict_mode;
@@ -1048,8 +1056,9 @@ static int run_steps(mm_engine *e)
   const int grid0 = launch_grid(e->act_n[0]), grid1 = launch_grid(e->act_n[1]);
   if (grid0 == 0 && grid1 == 0) return 0;
   if (!e->cfg.use_graph) return queue_steps(e, grid0, grid1);
-  const std::array<int, 8> key = {grid0, grid1, launch_grid(e->tile_n[0]), launch_grid(e->tile_n[1]),
-                                  e->a_lds[0], e->a_lds[1], e->a_slab[0], e->a_slab[1]};
+  const std::array<int, 10> key = {grid0, grid1, launch_grid(e->tile_n[0]), launch_grid(e->tile_n[1]),
+                                   e->a_lds[0], e->a_lds[1], e->a_slab[0], e->a_slab[1], e->s_big[0],
+                                   e->s_big[1]};
   auto it = e->graphs.find(key);
   if (it == e->graphs.end()) {
     MM_HIP(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));

@@ -1619,7 +1619,11 @@ __global__ __launch_bounds__(256) void k_step_b(EngPtrs P, SolveParams sp)
   const unsigned skip = S->done_in;
   const int nslab = R->nslab;
   double s1[K];
-  group_sum_cols<K>(slab_part(P, R->first_slab).p, nslab, P.part_stride, 0, s1, lds);
+  if (R->big) {
+    for (int k = 0; k < K; k++) s1[k] = R->pq[k];  // summed by kernel S
+  } else {
+    group_sum_cols<K>(slab_part(P, R->first_slab).p, nslab, P.part_stride, 0, s1, lds);
+  }
   if (tid == 0)
     for (int k = 0; k < K; k++)
       s_alpha[k] = (((skip >> k) & 1u) || s1[k] == 0.0) ? 0.0 : S->rz[k] / s1[k];
@@ -1877,19 +1881,18 @@ __device__ void r_slab(const EngPtrs &P, int slab, double (&lds)[RedLds<K>::size
   }
 }
 
-// Kernel R runs on a grid of at most kRGrid workgroups (launch_steps): in a
-// PCG sweep it only copies each RVE's state (first slab) and, for big RVEs,
-// sums r.r / r.z, so a workgroup per slab would mostly start and exit.
-// Workgroup b takes the positions j = b / 8 + m * (grid / 8) of its XCD's
-// part (b % 8) of the active list (the split of active_in).  Per round its
-// lanes look up one position each and collect the slabs with work (an RVE's
-// first slab, the residual and averaging phases) in LDS; the workgroup then
-// processes those, in any order (each writes only its own slab's outputs).
-template <typename TS, int K>
-__global__ __launch_bounds__(256) void k_step_r(EngPtrs P, SolveParams sp)
+// Runs work(slab) for the slabs of the active list for which want(slab, R)
+// holds, on a grid of at most kRGrid workgroups (kernels R and S, which have
+// work on few slabs in most launches, so a workgroup per slab would mostly
+// start and exit).  Workgroup b takes the positions j = b / 8 + m * (grid / 8)
+// of its XCD's part (b % 8) of the active list (the split of active_in).  Per
+// round its lanes look up one position each and collect the slabs with work
+// in LDS; the workgroup then works on those, in any order (each writes only
+// its own slab's outputs).
+template <typename Want, typename Work>
+__device__ void for_slabs_with_work(const EngPtrs &P, Want want, Work work)
 {
-  __shared__ double lds[RedLds<K>::size];
-  __shared__ int work[256];
+  __shared__ int slabs[256];
   __shared__ int nwork;
   const int *list = P.act_list;
   const int n = list[0];
@@ -1905,17 +1908,51 @@ __global__ __launch_bounds__(256) void k_step_r(EngPtrs P, SolveParams sp)
     if (m < mine && i < n) {
       const int slab = list[1 + i];
       const int slot = P.slab_rve[slab];
-      if (slot >= 0) {
-        const RveDev *R = P.rves + slot;
-        const int ph = R->st[1].phase;
-        if (slab == R->first_slab || ph == PH_RES || ph == PH_AVG) work[atomicAdd(&nwork, 1)] = slab;
-      }
+      if (slot >= 0 && want(slab, P.rves + slot)) slabs[atomicAdd(&nwork, 1)] = slab;
     }
     __syncthreads();
     const int cnt = nwork;
-    for (int w = 0; w < cnt; w++) r_slab<TS, K>(P, work[w], lds);
-    __syncthreads();  // work and nwork are rewritten by the next round
+    for (int w = 0; w < cnt; w++) work(slabs[w]);
+    __syncthreads();  // slabs and nwork are rewritten by the next round
   }
+}
+
+// Kernel R: an RVE's first slab passes the state on (and, for big RVEs in a
+// sweep, sums r.r / r.z); the residual and averaging phases work on every
+// slab.
+template <typename TS, int K>
+__global__ __launch_bounds__(256) void k_step_r(EngPtrs P, SolveParams sp)
+{
+  __shared__ double lds[RedLds<K>::size];
+  for_slabs_with_work(
+      P,
+      [](int slab, const RveDev *R) {
+        const int ph = R->st[1].phase;
+        return slab == R->first_slab || ph == PH_RES || ph == PH_AVG;
+      },
+      [&](int slab) { r_slab<TS, K>(P, slab, lds); });
+}
+
+// Kernel S, launched after A only when the launch's group holds a big RVE
+// (RveDev::big): sums A's p.q partials of each big RVE in a sweep once into
+// RveDev::pq (group_sum_cols over SN lanes, as every workgroup of B would),
+// so that kernel B reads K totals instead of re-summing nslab partials.
+template <typename TS, int K>
+__global__ __launch_bounds__(256) void k_step_s(EngPtrs P)
+{
+  __shared__ double lds[16 * K];
+  for_slabs_with_work(
+      P,
+      [](int slab, const RveDev *R) {
+        return R->big && slab == R->first_slab && R->st[0].phase == PH_INNER;
+      },
+      [&](int slab) {
+        RveDev *R = P.rves + P.slab_rve[slab];
+        double s1[K];
+        group_sum_cols<K>(slab_part(P, R->first_slab).p, R->nslab, P.part_stride, 0, s1, lds);
+        if (threadIdx.x == 0)
+          for (int k = 0; k < K; k++) R->pq[k] = s1[k];
+      });
 }
 
 // Stand-alone copy of the INNER-phase SpMV (w = A z) over every resident RVE,
@@ -1992,6 +2029,7 @@ static int launch_steps(const EngPtrs &P, const SolveParams &sp, int grid, int f
           }
         }
         if (P.a_lds == 0 || P.a_slab) k_step_a<TS, K, DICT><<<grid, P.SN, 0, s>>>(P, sp);
+        if (P.s_big) k_step_s<TS, K><<<grid < kRGrid ? grid : kRGrid, P.SN, 0, s>>>(P);
         break;
       case 1: k_step_b<TS, K><<<grid, P.SN, 0, s>>>(P, sp); break;
       default: k_step_r<TS, K><<<grid < kRGrid ? grid : kRGrid, P.SN, 0, s>>>(P, sp); break;

@@ -118,12 +118,14 @@ struct RveDev {
   // mnt * mncg <= nslab, is processed by the workgroup of slab
   // first_slab + t.  mcpp == 0: kernel A works slab by slab.
   int mcpp, mL, mnzc, mnt, mncg;
-  // big != 0 (nslab >= kBigSlabs): during PCG sweeps, kernel R sums B's r.r /
-  // r.z partials once into rr_rz (in its first slab's workgroup, in the
-  // order every workgroup of kernel A would use), and kernel A reads the
-  // totals instead of each workgroup re-summing nslab partials.
+  // big != 0 (nslab >= mm_engine_config::big_slabs): during PCG sweeps, kernel R sums B's r.r /
+  // r.z partials once into rr_rz and kernel S sums A's p.q partials once
+  // into pq (in the RVE's first slab's workgroup, in the order every
+  // workgroup of the consuming kernel would use), and kernels A and B read
+  // the totals instead of each workgroup re-summing nslab partials.
   int big;
   double rr_rz[2 * MM_MAX_RHS];
+  double pq[MM_MAX_RHS];
   double fx[MM_MAX_FIBERS], fy[MM_MAX_FIBERS], fr[MM_MAX_FIBERS];
   double young[kNmat], poisson[kNmat];
   double load[MM_MAX_RHS][MM_NVOI];
@@ -174,6 +176,7 @@ struct EngPtrs {
   int a_grid;    // grid of the plane march over act_tile (a_lds > 0)
   int a_lds;     // dynamic LDS bytes of the plane march; 0: march off
   int a_slab;    // march on: some RVEs (mcpp == 0) need k_step_a as well
+  int s_big;     // the group holds an RVE with big != 0: kernel S runs after A
 };
 
 
@@ -193,9 +196,10 @@ constexpr int kDictEntry = 27 * kDictBlk;
 // march of a 30^3 RVE (window 318 nodes) with its dictionary cache under
 // 40 KB, four workgroups per CU; three cases per round fit only three.
 constexpr int march_cases(int) { return 2; }
-// RVEs of at least this many slabs get their r.r / r.z totals from kernel R
-// (RveDev::big): below it the re-summing in kernel A costs less than the
-// extra latency of the single workgroup in R.
+// Default of mm_engine_config::big_slabs: RVEs of at least this many slabs
+// get their dot-product totals from kernels R and S (RveDev::big); below it
+// the re-summing in every workgroup costs less than the extra latency of
+// one workgroup in R and of kernel S's launch.
 constexpr int kBigSlabs = 1024;
 // Dictionary entries a march tile keeps in LDS, and the bytes of that cache
 // (entries, then one slot byte per pattern), a multiple of 16.

@@ -86,7 +86,8 @@ class Engine:
                  slab_nodes=0, rtol=1e-10, abs_tol=0.0, inner_rtol=1e-5, max_inner_its=20000,
                  max_sweeps=8, steps_per_launch=16, use_graph=True, sync_every_step=False,
                  device=0, ell_storage=_lib.MM_ELL_DICT, initial_guess=_lib.MM_GUESS_AFFINE,
-                 solver_streams=2, profile_step_kernel=False, march_planes=None):
+                 solver_streams=2, profile_step_kernel=False, march_planes=None,
+                 big_slabs=None):
         L = lib()
         c = EngineConfig()
         check(L.mm_engine_config_default(C.byref(c)), "mm_engine_config_default")
@@ -110,6 +111,8 @@ class Engine:
         c.profile_step_kernel = int(bool(profile_step_kernel))
         if march_planes is not None:
             c.march_planes = int(march_planes)
+        if big_slabs is not None:
+            c.big_slabs = int(big_slabs)
         self.cfg = c
         self.nrhs = nrhs
         h = C.c_void_p()
