@@ -206,12 +206,15 @@ constexpr int kBigSlabs = 1024;
 constexpr int kMarchDictCap = 10;
 constexpr int march_dict_bytes(int tsz) { return (kMarchDictCap * kDictEntry * tsz + kDictPat + 15) / 16 * 16; }
 // LDS of the plane march: the dictionary cache (dictionary storage), then a
-// ring of three planes of the window (SN + 2 nx + 2 nodes) for the round's
-// load cases, three rows each.
-inline int march_lds_bytes(int SN, int nx, int K, int tsz, bool dict)
+// ring of three planes of the window (the largest in-plane chunk of the even
+// split, plus nx + 1 nodes either side) for the round's load cases, three
+// rows each.
+inline int march_lds_bytes(int SN, int nx, int ny, int K, int tsz, bool dict)
 {
   const int g = K < march_cases(tsz) ? K : march_cases(tsz);
-  return (dict ? march_dict_bytes(tsz) : 0) + 3 * g * 3 * (SN + 2 * nx + 2) * tsz;
+  const long long nxy = (long long)nx * ny, cpp = (nxy + SN - 1) / SN;
+  const int chunk = (int)((nxy + cpp - 1) / cpp);  // the largest chunk of the even split
+  return (dict ? march_dict_bytes(tsz) : 0) + 3 * g * 3 * (chunk + 2 * nx + 2) * tsz;
 }
 constexpr int kMarchMaxLds = 64 * 1024;
 
