@@ -106,6 +106,11 @@ def test_argument_errors_without_gpu():
     assert cfg.solver_streams == 2
     cfg.solver_streams = 3  # one or two stream groups only
     assert L.mm_engine_create(C.byref(cfg), C.byref(h)) == _lib.MM_ERR_ARG
+    L.mm_engine_config_default(C.byref(cfg))
+    assert cfg.big_slabs == 1024 and cfg.march_planes == 8
+    cfg.big_slabs = 0  # per-RVE totals from a slab count of at least 1
+    assert L.mm_engine_create(C.byref(cfg), C.byref(h)) == _lib.MM_ERR_ARG
+    assert L.ell_device_set_pcg_fused(None, 4) == 1
     # a column id outside [-1, ncol) is refused before anything reaches the GPU
     e = Ell.from_arrays(np.array([[0, 3]], dtype=np.int32), np.array([[1.0, 2.0]]), ncol=2)
     assert L.ell_device_create(C.byref(e.m), 0, C.byref(h)) == 1
