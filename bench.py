@@ -410,14 +410,21 @@ def workload_c5(args, mm, configs):
 
 
 def cpu_model():
+    """lscpu's "Model name" verbatim, then its family / model numbers and
+    topology (a virtual machine may report only the brand in the name)."""
     try:
         out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
-        for ln in out.splitlines():
-            if ln.startswith("Model name:"):
-                return ln.split(":", 1)[1].strip()
     except (OSError, subprocess.SubprocessError):
-        pass
-    return "unknown CPU"
+        return "unknown CPU"
+    f = {}
+    for ln in out.splitlines():
+        if ":" in ln:
+            k, v = ln.split(":", 1)
+            f.setdefault(k.strip(), v.strip())
+    name = f.get("Model name", "unknown CPU")
+    extra = ["%s %s" % (k.lower(), f[k]) for k in ("CPU family", "Model", "Socket(s)", "Core(s) per socket",
+                                                    "Thread(s) per core") if k in f]
+    return "%s (lscpu: %s)" % (name, ", ".join(extra)) if extra else name
 
 
 def main():
