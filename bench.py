@@ -39,8 +39,9 @@ replicas and the streaming probes (the workload of the counter runs).
       mixed-precision stress against an FP64 solve and the oracle fixture
   c4  512 RVEs of uneven size (8^3..32^3), 6 load cases each: tangents/s
   ell the reference ELL API's device mirror (one 30^3 and one 100^3 RVE):
-      FP64 SpMV, and FP64 Jacobi-PCG ms per iteration with four launches
-      per iteration and with the persistent kernel
+      FP64 SpMV, and FP64 Jacobi-PCG ms per iteration with the per-step
+      kernels (two or four launches per iteration) and with the persistent
+      kernel
   c5  FE^2 serving stream (12^3/20^3/30^3 RVEs, strain + tangent per request,
       Poisson arrivals): a sweep of arrival rates (throughput, p50 / p99,
       the saturation knee), then the full 32,768-request stream with a
@@ -118,8 +119,8 @@ PCG_ROW_BYTES = 81 * 12 + 4 * 8 + 8 * 8
 
 def pcg_ms_per_iteration(mm, h, rows, fused, n1, n2):
     """ms per FP64 Jacobi-PCG iteration of ell_device_solve_pcg on mirror h
-    with `fused` iterations per persistent launch (0: four launches per
-    iteration): the slope between solves capped at n1 and n2 iterations
+    with `fused` iterations per persistent launch (0: the per-step kernels,
+    two or four launches per iteration): the slope between solves capped at n1 and n2 iterations
     (zero start, random right-hand side, a target no solve reaches), so the
     transfers of b and x and the set-up cancel.  Best of two per cap."""
     import ctypes as C
@@ -150,8 +151,8 @@ def ell_api_spmv(mm, spec, reps=20, pcg_its=(60, 260)):
     matrix of one RVE, assembled on the GPU by an FP64 engine (no solve).
     Bytes per launch: N * 81 * (8 + 4) + 16 N (matrix, x once, y).  Then the
     FP64 Jacobi-PCG of ell_device_solve_pcg on the same mirror, ms per
-    iteration with four launches per iteration and with the persistent
-    kernel at 1, 32 and all iterations per launch."""
+    iteration with the per-step kernels and with the persistent kernel at 1,
+    32 and all iterations per launch."""
     import ctypes as C
     rows = 3 * (spec.nex + 1) * (spec.ney + 1) * (spec.nez + 1)
     with mm.Engine(nrhs=1, precision=mm.MM_PREC_FP64, hbm_budget=int(rows * 81 * 16 * 1.3) + (256 << 20),
@@ -172,7 +173,10 @@ def ell_api_spmv(mm, spec, reps=20, pcg_its=(60, 260)):
         mm._lib.check(L.ell_device_time_mvp(h, reps, C.byref(ms)), "ell_device_time_mvp")
         if pcg_its:
             pcg = {"bytes_per_iteration": rows * PCG_ROW_BYTES}
-            for name, f in (("four_launches", 0), ("fused_1", 1), ("fused_32", 32), ("fused_all", 1 << 30)):
+            # per-step kernels: two launches per iteration up to 1024 slabs of
+            # 256 rows (every workgroup sums the partials), four beyond
+            pcg["per_step_launches_per_iteration"] = 2 if -(-rows // 256) <= 1024 else 4
+            for name, f in (("per_step", 0), ("fused_1", 1), ("fused_32", 32), ("fused_all", 1 << 30)):
                 t = pcg_ms_per_iteration(mm, h, rows, f, *pcg_its)
                 pcg[name] = dict(ms_per_it=t, gbs=rows * PCG_ROW_BYTES / (t * 1e-3) / 1e9)
                 log("fp64 pcg (%s), %d rows: %.4f ms/iteration -> %.0f GB/s"
@@ -288,7 +292,7 @@ def workload_c4(args, mm, configs):
 def workload_ell(args, mm, configs):
     """The reference ELL API's device mirror on one 30^3 RVE of the batch and
     on the 100^3 RVE: the bit-exact FP64 SpMV and the FP64 Jacobi-PCG per
-    iteration, four launches per iteration against the persistent kernel."""
+    iteration, the per-step kernels against the persistent kernel."""
     out = {"workload": "ell: reference-layout FP64 ELL of one 30^3 and one 100^3 RVE, SpMV and Jacobi-PCG"}
     out["rve_30"] = ell_api_spmv(mm, configs.c2_spec(2, 0, n=30))
     out["rve_100"] = ell_api_spmv(mm, configs.c3_spec(), pcg_its=(20, 100))

@@ -106,14 +106,15 @@ int ell_device_solve_pcg(ell_device *d, ell_solver *solver, const double *b, dou
 int ell_device_solve_jacobi(ell_device *d, ell_solver *solver, const double *b, double *x);
 int ell_device_destroy(ell_device *d);
 /* How ell_device_solve_pcg (and the _noalloc variant) runs its iterations on
- * this mirror.  0: four launches per iteration (q = A p, the p.q reduction,
- * the vector update, the r.r / r.z reduction).  n > 0: one cooperative
- * persistent kernel that runs up to n whole iterations per launch, its
- * workgroups meeting at two grid barriers per iteration; the host checks for
- * convergence between launches.  Both give the same bits.  Returns 1 for
- * n < 0, or for n > 0 on a mirror whose slabs are not 256 rows or on a device
- * without cooperative launch.  No counterpart in the reference: it selects
- * how src/solvers.c's PCG loop is mapped onto kernels. */
+ * this mirror.  0 (the default): two launches per iteration, q = A p and the
+ * vector update, each of whose workgroups first sums the other's slab
+ * partials itself.  n > 0: one cooperative persistent kernel that runs up to
+ * n whole iterations per launch, its workgroups meeting at two grid barriers
+ * per iteration.  In both, the host checks for convergence between launches.
+ * With 256-row slabs both give the same bits.  Returns 1 for n < 0, or for
+ * n > 0 on a mirror whose slabs are not 256 rows or on a device without
+ * cooperative launch.  No counterpart in the reference: it selects how the
+ * PCG loop is mapped onto kernels. */
 int ell_device_set_pcg_fused(ell_device *d, int its_per_launch);
 
 /* No counterpart in the reference.  For callers that cannot allocate device

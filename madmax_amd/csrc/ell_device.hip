@@ -33,14 +33,35 @@ struct IterState {
   double bnorm, rnorm, rz, alpha, beta, err;
   int its, done, max_its, pbuf;
   double rtol, min_err;
-  int pad0, pad1, pad2, pad3;
+  int pad0, pad1, pad2;  // k_pcg_fused: fault flag, and the counts a failed barrier wait saw
+  int fresh;             // PCG: 1 from the start until the first update's partials exist
 };
 
 // Dot products: every workgroup of a big kernel writes its partial sums
-// (part[slab * 4 + i]); a one-workgroup kernel then sums them in slab order
-// and updates IterState, which the big kernels only read.  Kernel
-// boundaries order the two, so no workgroup fences for another.
+// (part[slab * 4 + i]).  Jacobi and the PCG start sum them in a one-workgroup
+// kernel (k_reduce) in slab order; in the PCG iteration every workgroup of
+// the consuming kernel sums them itself, in the same order, so all hold the
+// same scalars.  Kernel boundaries order producer and consumer, so no
+// workgroup fences for another.
 enum RedMode : int { RED_JAC = 0, RED_INIT = 1, RED_PQ = 2, RED_RR = 3 };
+
+// The scalar steps of Jacobi-PCG, shared by every kernel that takes them so
+// that they round alike.  After q = A p: alpha from p.q.  After the update:
+// count the iteration, test ||r|| <= rtol ||b|| or the cap, else beta and
+// the new r.z.
+__device__ inline double pcg_alpha(double rz, double pq) { return rz / pq; }
+__device__ inline void pcg_after_update(double rr, double rzn, double rtol, double bnorm, int max_its, int &its,
+                                        double &rnorm, double &rz, double &beta, int &done)
+{
+  its += 1;
+  rnorm = sqrt(rr);
+  if (rnorm <= rtol * bnorm || its >= max_its) {
+    done = 1;
+  } else {
+    beta = rzn / rz;
+    rz = rzn;
+  }
+}
 
 // Row `lane` of the slab times x: ascending slots, stop at the first -1,
 // separate multiply and add.  Slots are fetched kU at a time (column ids and
@@ -49,6 +70,11 @@ enum RedMode : int { RED_JAC = 0, RED_INIT = 1, RED_PQ = 2, RED_RR = 3 };
 // other.  Column ids are -1 or in [0, ncol) (checked on upload), so the
 // entries fetched past a row's first -1 are in bounds and are never added.
 constexpr int kU = 8;
+// Matrices of at most this many slabs sum the PCG partials in every
+// workgroup (two launches per iteration); larger ones reduce them in
+// k_reduce launches (four).  On one MI355X: 350 slabs (30^3 RVE) 0.042 ms
+// against 0.045 ms per iteration; 12,070 slabs (100^3) 0.89 against 0.73.
+constexpr int kSelfSumSlabs = 1024;
 __device__ inline double row_dot_exact(const int *__restrict__ cols, const double *__restrict__ vals,
                                        long long slab_base, int S, int nnz, int lane,
                                        const double *__restrict__ x)
@@ -119,21 +145,15 @@ __global__ __launch_bounds__(256) void k_reduce(const double *__restrict__ part,
       st->beta = 0.0;
       st->its = 0;
       st->pbuf = 0;
+      st->fresh = 1;
       st->done = (st->bnorm == 0.0 || st->rnorm <= st->rtol * st->bnorm) ? 1 : 0;
       break;
     case RED_PQ:
-      st->alpha = st->rz / s[0];
+      st->alpha = pcg_alpha(st->rz, s[0]);
       st->pbuf ^= 1;
       break;
-    default:  // RED_RR
-      st->its += 1;
-      st->rnorm = sqrt(s[0]);
-      if (st->rnorm <= st->rtol * st->bnorm || st->its >= st->max_its) {
-        st->done = 1;
-      } else {
-        st->beta = s[1] / st->rz;
-        st->rz = s[1];
-      }
+    default:  // RED_RR: r.r and r.z are k_pcg_update's part[.][1..2]
+      pcg_after_update(s[1], s[2], st->rtol, st->bnorm, st->max_its, st->its, st->rnorm, st->rz, st->beta, st->done);
   }
 }
 
@@ -264,48 +284,87 @@ __device__ inline void pcg_update_row(long long row, double alpha, const double 
   v[1] = ri * zi;
 }
 
-// q = A p (pcg_q_row); partial p.q (part[.][0])
-__global__ void k_pcg_spmv(const int *__restrict__ cols, const double *__restrict__ vals, int nrow,
-                           int nnz, int S, const double *__restrict__ z, double *__restrict__ pA,
-                           double *__restrict__ pB, double *__restrict__ q,
-                           double *__restrict__ part, const IterState *st)
+// Jacobi-PCG as two launches per iteration.  The state is kept in two
+// copies: k_pcg_spmv reads st_in (written by the PCG start or the last
+// update) and writes st_out, k_pcg_update the reverse, so that no launch
+// reads a copy it writes.  Workgroup 0 writes the copy.  With self_sum every
+// workgroup forms the scalars from the other kernel's slab partials itself
+// (the same values in every workgroup); without it (matrices of many slabs,
+// where nslab workgroups reading nslab partials each would cost more than a
+// launch) a k_reduce launch before each kernel has already applied them to
+// st_in.
+
+// First the update's r.r / r.z (part[.][1..2]) give the iteration count, the
+// convergence test and beta (nothing to sum right after the start); then
+// q = A p (pcg_q_row) with partial p.q (part[.][0]).
+__global__ __launch_bounds__(256) void k_pcg_spmv(const int *__restrict__ cols, const double *__restrict__ vals,
+                                                  int nrow, int nnz, int S, int nslab,
+                                                  const double *__restrict__ z, double *__restrict__ pA,
+                                                  double *__restrict__ pB, double *__restrict__ q,
+                                                  double *__restrict__ part, const IterState *st_in,
+                                                  IterState *st_out, int self_sum)
 {
-  __shared__ double lds[16];
-  if (st->done) return;
-  const double beta = st->beta;
-  const double *pold = st->pbuf ? pB : pA;
-  double *pnew = st->pbuf ? pA : pB;
+  __shared__ double lds[48];
+  __shared__ IterState N;
   const int lane = threadIdx.x;
+  if (lane == 0) N = *st_in;
+  __syncthreads();
+  if (self_sum && !N.done && !N.fresh) {
+    double s[2];
+    group_sum_rows<2>(part + 1, nslab, 4, s, lds);
+    if (lane == 0) pcg_after_update(s[0], s[1], N.rtol, N.bnorm, N.max_its, N.its, N.rnorm, N.rz, N.beta, N.done);
+    __syncthreads();
+  }
+  if (blockIdx.x == 0 && lane == 0) {
+    IterState o = N;
+    o.fresh = 0;
+    *st_out = o;
+  }
+  if (N.done) return;
+  const double beta = N.beta;
+  const double *pold = N.pbuf ? pB : pA;
+  double *pnew = N.pbuf ? pA : pB;
   const long long row = (long long)blockIdx.x * S + lane;
   double v[1] = {0.0};
   if (row < nrow) v[0] = pcg_q_row(cols, vals, nnz, S, blockIdx.x, lane, row, z, beta, pold, pnew, q);
   block_sum<1>(v, lds);
-  if (threadIdx.x == 0) {
-    part[blockIdx.x * 4] = v[0];
-    part[blockIdx.x * 4 + 1] = 0.0;
-    part[blockIdx.x * 4 + 2] = 0.0;
-  }
+  if (lane == 0) part[blockIdx.x * 4] = v[0];
 }
 
-// x += alpha p, r -= alpha q, z = dinv r; partials r.r, r.z (part[.][0..1])
-__global__ void k_pcg_update(int nrow, int S, const double *__restrict__ dinv,
-                             const double *__restrict__ pA, const double *__restrict__ pB,
-                             const double *__restrict__ q, double *__restrict__ x,
-                             double *__restrict__ r, double *__restrict__ z,
-                             double *__restrict__ part, const IterState *st)
+// alpha from the p.q partials; x += alpha p, r -= alpha q, z = dinv r
+// (pcg_update_row); partials r.r, r.z (part[.][1..2]).
+__global__ __launch_bounds__(256) void k_pcg_update(int nrow, int S, int nslab, const double *__restrict__ dinv,
+                                                    const double *__restrict__ pA, const double *__restrict__ pB,
+                                                    const double *__restrict__ q, double *__restrict__ x,
+                                                    double *__restrict__ r, double *__restrict__ z,
+                                                    double *__restrict__ part, const IterState *st_in,
+                                                    IterState *st_out, int self_sum)
 {
   __shared__ double lds[32];
-  if (st->done) return;
-  const double alpha = st->alpha;
-  const double *p = st->pbuf ? pB : pA;  // buffer written by k_pcg_spmv
-  const long long row = (long long)blockIdx.x * S + threadIdx.x;
+  __shared__ IterState N;
+  const int lane = threadIdx.x;
+  if (lane == 0) N = *st_in;
+  __syncthreads();
+  if (self_sum && !N.done) {
+    double s[1];
+    group_sum_rows<1>(part, nslab, 4, s, lds);
+    if (lane == 0) {
+      N.alpha = pcg_alpha(N.rz, s[0]);
+      N.pbuf ^= 1;
+    }
+    __syncthreads();
+  }
+  if (blockIdx.x == 0 && lane == 0) *st_out = N;
+  if (N.done) return;
+  const double alpha = N.alpha;
+  const double *p = N.pbuf ? pB : pA;  // the buffer k_pcg_spmv wrote
+  const long long row = (long long)blockIdx.x * S + lane;
   double v[2] = {0.0, 0.0};
   if (row < nrow) pcg_update_row(row, alpha, dinv, p, q, x, r, z, v);
   block_sum<2>(v, lds);
-  if (threadIdx.x == 0) {
-    part[blockIdx.x * 4] = v[0];
-    part[blockIdx.x * 4 + 1] = v[1];
-    part[blockIdx.x * 4 + 2] = 0.0;
+  if (lane == 0) {
+    part[blockIdx.x * 4 + 1] = v[0];
+    part[blockIdx.x * 4 + 2] = v[1];
   }
 }
 
@@ -315,10 +374,10 @@ __global__ void k_pcg_update(int nrow, int S, const double *__restrict__ dinv,
 // blockIdx.x, +G, ...).  An iteration is q = A p for the own slabs, a grid
 // barrier, p.q summed by every workgroup from the slab partials, the vector
 // update of the own slabs, a second barrier and the r.r / r.z sums.  Every
-// workgroup sums the same partials in the same order as k_reduce (256 lanes,
-// slab-strided, then block_sum), so all of them hold the same scalars, take
-// the same branch at the convergence test and match the per-step kernels bit
-// for bit.  Slab partials: p.q in part[.][0], r.r and r.z in part[.][1..2];
+// workgroup sums the same partials in the same order as the two-launch
+// kernels (256 lanes, slab-strided, then block_sum) and takes the same scalar
+// steps, so all of them hold the same scalars, take the same branch at the
+// convergence test and match k_pcg_spmv / k_pcg_update bit for bit.  Slab partials: p.q in part[.][0], r.r and r.z in part[.][1..2];
 // each is rewritten only after a barrier that every reader of the previous
 // value has passed.
 constexpr int kFusedLanes = 256;
@@ -422,7 +481,7 @@ __global__ __launch_bounds__(kFusedLanes) void k_pcg_fused(
     group_sum_rows<1>(part, nslab, 4, s1, lds);
     if (lane == 0) bc[0] = s1[0];
     __syncthreads();
-    alpha = rz / bc[0];
+    alpha = pcg_alpha(rz, bc[0]);
     pbuf ^= 1;
     const double *p = pbuf ? pB : pA;  // = pnew
     for (int slab = blockIdx.x; slab < nslab; slab += G) {
@@ -442,14 +501,7 @@ __global__ __launch_bounds__(kFusedLanes) void k_pcg_fused(
       bc[1] = s[1];
     }
     __syncthreads();
-    n += 1;
-    rnorm = sqrt(bc[0]);
-    if (rnorm <= rtol * bnorm || n >= max_its) {
-      done = 1;
-    } else {
-      beta = bc[1] / rz;
-      rz = bc[1];
-    }
+    pcg_after_update(bc[0], bc[1], rtol, bnorm, max_its, n, rnorm, rz, beta, done);
     __syncthreads();  // bc is rewritten by the next iteration
   }
   // Every workgroup read the state before its first arrival, and workgroup 0
@@ -462,6 +514,7 @@ __global__ __launch_bounds__(kFusedLanes) void k_pcg_fused(
     st->beta = beta;
     st->pbuf = pbuf;
     st->done = done;
+    st->fresh = 0;
   }
 }
 
@@ -481,8 +534,8 @@ __global__ void k_zero(double *__restrict__ x, int n)
 // PCG work vectors of one solve
 struct PcgVecs {
   double *r, *z, *pA, *pB, *q, *part;
-  IterState *st;
-  unsigned *bar;  // grid-barrier counter of k_pcg_fused
+  IterState *st;   // two copies (the two-launch iteration passes the state between them)
+  unsigned *bar;   // grid-barrier words of k_pcg_fused
 };
 
 }  // namespace
@@ -495,7 +548,7 @@ struct ell_device_ {
   double *x, *b, *r, *z, *pA, *pB, *q, *part;
   IterState *st;
   unsigned *bar;
-  int fused_its;   // 0: four launches per PCG iteration; n > 0: k_pcg_fused, n iterations per launch
+  int fused_its;   // 0: two launches per PCG iteration; n > 0: k_pcg_fused, n iterations per launch
   int fused_grid;  // co-resident workgroups of k_pcg_fused (set with fused_its)
   hipStream_t stream;
 };
@@ -599,10 +652,10 @@ extern "C" int ell_device_create(const ell_matrix *m, int slab_rows, ell_device 
   ALLOC(d->pB, nvc * sizeof(double));
   ALLOC(d->q, nv * sizeof(double));
   ALLOC(d->part, (size_t)(d->nslab + 1) * 4 * sizeof(double));
-  ALLOC(d->st, sizeof(IterState));
+  ALLOC(d->st, 2 * sizeof(IterState));
   ALLOC(d->bar, kBarBytes);
 #undef ALLOC
-  if (hipMemsetAsync(d->st, 0, sizeof(IterState), d->stream) != hipSuccess) {
+  if (hipMemsetAsync(d->st, 0, 2 * sizeof(IterState), d->stream) != hipSuccess) {
     rc = MM_ERR_HIP;
     goto fail;
   }
@@ -770,14 +823,17 @@ static int pcg_run(ell_device *d, const PcgVecs &V, ell_solver *solver, const do
       return MM_ERR_HIP;
     }
   }
+  // the state passes st[0] -> st[1] -> st[0]; up to kSelfSumSlabs slabs two
+  // launches per iteration, beyond that a one-workgroup k_reduce before each
+  const int self_sum = d->nslab <= kSelfSumSlabs ? 1 : 0;
   while (!h.done) {
     for (int k = 0; k < 32; k++) {
-      k_pcg_spmv<<<d->nslab, d->S, 0, d->stream>>>(d->cols, d->vals, d->nrow, d->nnz, d->S, V.z, V.pA, V.pB,
-                                                   V.q, V.part, V.st);
-      k_reduce<<<1, 256, 0, d->stream>>>(V.part, d->nslab, RED_PQ, V.st);
-      k_pcg_update<<<d->nslab, d->S, 0, d->stream>>>(d->nrow, d->S, d->dinv, V.pA, V.pB, V.q, x, V.r, V.z,
-                                                     V.part, V.st);
-      k_reduce<<<1, 256, 0, d->stream>>>(V.part, d->nslab, RED_RR, V.st);
+      k_pcg_spmv<<<d->nslab, d->S, 0, d->stream>>>(d->cols, d->vals, d->nrow, d->nnz, d->S, d->nslab, V.z, V.pA,
+                                                   V.pB, V.q, V.part, V.st, V.st + 1, self_sum);
+      if (!self_sum) k_reduce<<<1, 256, 0, d->stream>>>(V.part, d->nslab, RED_PQ, V.st + 1);
+      k_pcg_update<<<d->nslab, d->S, 0, d->stream>>>(d->nrow, d->S, d->nslab, d->dinv, V.pA, V.pB, V.q, x, V.r,
+                                                     V.z, V.part, V.st + 1, V.st, self_sum);
+      if (!self_sum) k_reduce<<<1, 256, 0, d->stream>>>(V.part, d->nslab, RED_RR, V.st);
     }
     MM_HIP(hipGetLastError());
     rc = get_state(d, V.st, &h);
@@ -870,12 +926,12 @@ int mm::ell_device_alloc(int nrow, int nnz, int S, ell_device **out)
   ALLOC(d->pB, nv * sizeof(double));
   ALLOC(d->q, nv * sizeof(double));
   ALLOC(d->part, (size_t)(d->nslab + 1) * 4 * sizeof(double));
-  ALLOC(d->st, sizeof(IterState));
+  ALLOC(d->st, 2 * sizeof(IterState));
   ALLOC(d->bar, kBarBytes);
 #undef ALLOC
   if (hipMemsetAsync(d->cols, 0xff, slots * sizeof(int), d->stream) != hipSuccess ||
       hipMemsetAsync(d->vals, 0, slots * sizeof(double), d->stream) != hipSuccess ||
-      hipMemsetAsync(d->st, 0, sizeof(IterState), d->stream) != hipSuccess ||
+      hipMemsetAsync(d->st, 0, 2 * sizeof(IterState), d->stream) != hipSuccess ||
       hipStreamSynchronize(d->stream) != hipSuccess) {
     rc = MM_ERR_HIP;
     goto fail;
@@ -908,13 +964,14 @@ int mm::ell_device_pcg_device(ell_device *d, ell_solver *solver, const double *b
 // The PCG work vectors, partial sums and iteration state of
 // ell_device_solve_pcg_noalloc live in this file-scope array: five vectors of
 // ELL_NOALLOC_MAX_ROWS doubles, the partials of up to ELL_NOALLOC_MAX_ROWS / 64
-// slabs, one IterState and the fused kernel's barrier counter (about 40.5 MiB,
-// held on every GPU the library runs on until the process ends).  Host code never uses the array's name as an
-// address: the per-device address comes from hipGetSymbolAddress.
+// slabs, two IterState copies and the fused kernel's barrier words (about
+// 40.5 MiB, held on every GPU the library runs on until the process ends).
+// Host code never uses the array's name as an address: the per-device
+// address comes from hipGetSymbolAddress.
 namespace {
 constexpr size_t kNaRows = ELL_NOALLOC_MAX_ROWS;
 constexpr size_t kNaPart = (kNaRows / 64 + 1) * 4;
-constexpr size_t kNaState = (sizeof(IterState) + 7) / 8;
+constexpr size_t kNaState = (2 * sizeof(IterState) + 7) / 8;
 constexpr size_t kNaBar = kBarBytes / 8;  // the grid-barrier words of k_pcg_fused
 constexpr size_t kNaDoubles = 5 * kNaRows + kNaPart + kNaState + kNaBar;
 constexpr int kNaMaxDevices = 64;

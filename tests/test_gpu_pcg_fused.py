@@ -1,6 +1,6 @@
 """The fused persistent Jacobi-PCG (ell_device_set_pcg_fused): one cooperative
 kernel that runs whole iterations, its workgroups meeting at two grid
-barriers per iteration, must give the bits of the four-launch iteration
+barriers per iteration, must give the bits of the two-launch iteration
 (same row arithmetic, same reduction order) for any number of iterations per
 launch, on systems where every workgroup owns one slab and on one where
 workgroups own several (more slabs than co-resident workgroups)."""
@@ -70,7 +70,7 @@ class Mirror:
 
 
 @pytest.mark.parametrize("per_launch", [1, 5, 100000])
-def test_fused_equals_four_launch_iteration_on_rves(per_launch):
+def test_fused_equals_two_launch_iteration_on_rves(per_launch):
     for n, eps in ((10, np.array([1e-3, 0, 2e-4, 0, 1e-4, 0])), (7, np.array([0, -1e-3, 0, 3e-4, 0, 0]))):
         r, cols, vals, b = _rve_system(n, eps)
         print("fused pcg: rve", n, "rows", cols.shape[0], "per launch", per_launch, flush=True)
@@ -90,7 +90,7 @@ def test_fused_equals_four_launch_iteration_on_rves(per_launch):
         assert np.max(np.abs(x1 - x_ref)) <= 1e-9 * np.max(np.abs(x_ref))
 
 
-def test_fused_equals_four_launch_iteration_with_several_slabs_per_workgroup():
+def test_fused_equals_two_launch_iteration_with_several_slabs_per_workgroup():
     cols, vals = _laplace_3d(85)  # 614,125 rows = 2,399 slabs of 256
     rng = np.random.default_rng(5)
     b = rng.standard_normal(cols.shape[0])
@@ -132,7 +132,7 @@ def test_fused_setting_is_checked():
         m.close()
 
 
-def test_fused_noalloc_equals_four_launch_allocating():
+def test_fused_noalloc_equals_two_launch_allocating():
     """The no-allocation entry point on a fused mirror: its work vectors and
     the barrier counter come from the library's file-scope array."""
     _, cols, vals, b = _rve_system(8, np.array([0, 0, 1e-3, 0, 0, 2e-4]))
