@@ -284,7 +284,8 @@ __device__ inline void pcg_update_row(long long row, double alpha, const double 
   v[1] = ri * zi;
 }
 
-// Jacobi-PCG as two launches per iteration.  The state is kept in two
+// Jacobi-PCG as two launches per iteration (one lane per row: S lanes, up to
+// 1024, per workgroup).  The state is kept in two
 // copies: k_pcg_spmv reads st_in (written by the PCG start or the last
 // update) and writes st_out, k_pcg_update the reverse, so that no launch
 // reads a copy it writes.  Workgroup 0 writes the copy.  With self_sum every
@@ -297,7 +298,7 @@ __device__ inline void pcg_update_row(long long row, double alpha, const double 
 // First the update's r.r / r.z (part[.][1..2]) give the iteration count, the
 // convergence test and beta (nothing to sum right after the start); then
 // q = A p (pcg_q_row) with partial p.q (part[.][0]).
-__global__ __launch_bounds__(256) void k_pcg_spmv(const int *__restrict__ cols, const double *__restrict__ vals,
+__global__ void k_pcg_spmv(const int *__restrict__ cols, const double *__restrict__ vals,
                                                   int nrow, int nnz, int S, int nslab,
                                                   const double *__restrict__ z, double *__restrict__ pA,
                                                   double *__restrict__ pB, double *__restrict__ q,
@@ -333,7 +334,7 @@ __global__ __launch_bounds__(256) void k_pcg_spmv(const int *__restrict__ cols, 
 
 // alpha from the p.q partials; x += alpha p, r -= alpha q, z = dinv r
 // (pcg_update_row); partials r.r, r.z (part[.][1..2]).
-__global__ __launch_bounds__(256) void k_pcg_update(int nrow, int S, int nslab, const double *__restrict__ dinv,
+__global__ void k_pcg_update(int nrow, int S, int nslab, const double *__restrict__ dinv,
                                                     const double *__restrict__ pA, const double *__restrict__ pB,
                                                     const double *__restrict__ q, double *__restrict__ x,
                                                     double *__restrict__ r, double *__restrict__ z,

@@ -91,6 +91,33 @@ def test_ell_solve_pcg_matches_oracle_c1():
     assert np.max(np.abs(x - x_ref)) <= 1e-9 * np.max(np.abs(x_ref))
 
 
+@pytest.mark.parametrize("slab_rows", [64, 128, 512, 1024])
+def test_ell_device_pcg_any_slab_size(slab_rows):
+    """The device mirror's PCG with slabs of 64 to 1024 rows (one lane per
+    row, so up to 1024 lanes per workgroup): the oracle's solution and
+    iteration count, as with the default 256-row slabs."""
+    import ctypes as C
+
+    from madmax_amd._lib import EllSolver
+    r, cols, vals, b = _c1_system()
+    x_ref, its_ref, _, ierr = po.pcg(r.ell(), b, rtol=1e-10)
+    assert ierr == 0
+    e = Ell.from_arrays(cols, vals)
+    h = C.c_void_p()
+    print("device pcg, slab rows", slab_rows, flush=True)
+    assert lib().ell_device_create(C.byref(e.m), slab_rows, C.byref(h)) == 0
+    try:
+        x = np.zeros_like(b)
+        s = EllSolver(100000, 0, 1e-10, 0.0)
+        rc = lib().ell_device_solve_pcg(h, C.byref(s), b.ctypes.data_as(C.POINTER(C.c_double)),
+                                        x.ctypes.data_as(C.POINTER(C.c_double)))
+    finally:
+        lib().ell_device_destroy(h)
+        e.free()
+    assert rc == 0 and abs(s.its - its_ref) <= 1
+    assert np.max(np.abs(x - x_ref)) <= 1e-9 * np.max(np.abs(x_ref))
+
+
 def test_ell_solve_pcg_zero_rhs_and_cap():
     _, cols, vals, b = _c1_system()
     e = Ell.from_arrays(cols, vals)
