@@ -46,8 +46,6 @@ int upload_slot_tables(hipStream_t s)
   return 0;
 }
 
-constexpr int kPart = kPartDoubles;
-
 // LDS scratch of the solver kernels: 16 doubles per reduced value, for the
 // largest reduction a kernel performs (4K residual norms, 12 averages).
 template <int K>
