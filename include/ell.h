@@ -123,7 +123,7 @@ int ell_device_set_pcg_fused(ell_device *d, int its_per_launch);
  * same bits) on a mirror created beforehand, with b and x (initial guess in,
  * solution out) in device memory the caller owns.  The PCG work vectors,
  * partial sums and iteration state live in one library-wide device array
- * sized for ELL_NOALLOC_MAX_ROWS rows (about 40.5 MiB per GPU, held until the
+ * sized for ELL_NOALLOC_MAX_ROWS rows (about 48.5 MiB per GPU, held until the
  * process ends); the call allocates nothing.  Returns 1 for a matrix with more
  * rows, 0 when converged, 5 at max_its.  Every call on one GPU uses the same
  * array: calls that overlap on one GPU race. */

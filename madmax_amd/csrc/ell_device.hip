@@ -201,8 +201,7 @@ __global__ void k_jac_upd(int nrow, int S, const double *__restrict__ dinv,
 __global__ void k_pcg_init(const int *__restrict__ cols, const double *__restrict__ vals, int nrow,
                            int nnz, int S, const double *__restrict__ x, const double *__restrict__ b,
                            const double *__restrict__ dinv, double *__restrict__ r,
-                           double *__restrict__ z, double *__restrict__ p0, double *__restrict__ p1,
-                           double *__restrict__ part)
+                           double2 *__restrict__ zp0, double2 *__restrict__ zp1, double *__restrict__ part)
 {
   __shared__ double lds[48];
   const int lane = threadIdx.x;
@@ -215,9 +214,8 @@ __global__ void k_pcg_init(const int *__restrict__ cols, const double *__restric
     const double ri = bi - ax;
     const double zi = dinv[row] * ri;
     r[row] = ri;
-    z[row] = zi;
-    p0[row] = 0.0;
-    p1[row] = 0.0;
+    zp0[row] = make_double2(zi, 0.0);
+    zp1[row] = make_double2(0.0, 0.0);
     v[0] = bi * bi;
     v[1] = ri * ri;
     v[2] = ri * zi;
@@ -227,14 +225,19 @@ __global__ void k_pcg_init(const int *__restrict__ cols, const double *__restric
     for (int i = 0; i < 3; i++) part[blockIdx.x * 4 + i] = v[i];
 }
 
+// z and p are stored as pairs {z, p} per row, in two buffers that the
+// iterations alternate between (the state's pbuf): q = A p reads the current
+// buffer's pairs (one 16-byte gather per column) and writes the new p into
+// the other buffer, whose z the update then writes.
+
 // Row `row` (lane `lane` of slab `slab`) of q = A p with p = z + beta p_old
-// formed on the fly, both for the own row, which is stored, and for every
-// gathered column; returns q_row * p_row.  Shared by the per-step kernels and
-// the fused kernel, so both do the same arithmetic.
+// formed on the fly from the current pairs zc, both for the own row, whose p
+// is stored in zn, and for every gathered column; returns q_row * p_row.
+// Shared by the per-step kernels and the fused kernel, so both do the same
+// arithmetic.
 __device__ inline double pcg_q_row(const int *__restrict__ cols, const double *__restrict__ vals, int nnz, int S,
-                                   int slab, int lane, long long row, const double *__restrict__ z, double beta,
-                                   const double *__restrict__ pold, double *__restrict__ pnew,
-                                   double *__restrict__ q)
+                                   int slab, int lane, long long row, const double2 *__restrict__ zc, double beta,
+                                   double2 *__restrict__ zn, double *__restrict__ q)
 {
   const long long base = (long long)slab * S * nnz;
   double acc = 0.0;
@@ -250,7 +253,10 @@ __device__ inline double pcg_q_row(const int *__restrict__ cols, const double *_
       v[u] = in ? __builtin_nontemporal_load(&vals[e]) : 0.0;
     }
 #pragma unroll
-    for (int u = 0; u < kU; u++) t[u] = (c[u] >= 0) ? z[c[u]] + beta * pold[c[u]] : 0.0;
+    for (int u = 0; u < kU; u++) {
+      const double2 w = (c[u] >= 0) ? zc[c[u]] : make_double2(0.0, 0.0);
+      t[u] = w.x + beta * w.y;
+    }
     bool stop = false;
 #pragma unroll
     for (int u = 0; u < kU; u++) {
@@ -262,24 +268,25 @@ __device__ inline double pcg_q_row(const int *__restrict__ cols, const double *_
     }
     if (stop) break;
   }
-  const double pi = z[row] + beta * pold[row];
-  pnew[row] = pi;
+  const double2 w = zc[row];
+  const double pi = w.x + beta * w.y;
+  zn[row].y = pi;
   q[row] = acc;
   return pi * acc;
 }
 
-// Row `row` of x += alpha p, r -= alpha q, z = dinv r; v = {r.r, r.z} terms.
+// Row `row` of x += alpha p, r -= alpha q, z = dinv r, with p and the new z
+// in the pairs zp; v = {r.r, r.z} terms.
 __device__ inline void pcg_update_row(long long row, double alpha, const double *__restrict__ dinv,
-                                      const double *__restrict__ p, const double *__restrict__ q,
-                                      double *__restrict__ x, double *__restrict__ r, double *__restrict__ z,
-                                      double (&v)[2])
+                                      double2 *__restrict__ zp, const double *__restrict__ q,
+                                      double *__restrict__ x, double *__restrict__ r, double (&v)[2])
 {
-  const double xi = x[row] + alpha * p[row];
+  const double xi = x[row] + alpha * zp[row].y;
   const double ri = r[row] - alpha * q[row];
   const double zi = dinv[row] * ri;
   x[row] = xi;
   r[row] = ri;
-  z[row] = zi;
+  zp[row].x = zi;
   v[0] = ri * ri;
   v[1] = ri * zi;
 }
@@ -299,9 +306,8 @@ __device__ inline void pcg_update_row(long long row, double alpha, const double 
 // convergence test and beta (nothing to sum right after the start); then
 // q = A p (pcg_q_row) with partial p.q (part[.][0]).
 __global__ void k_pcg_spmv(const int *__restrict__ cols, const double *__restrict__ vals,
-                                                  int nrow, int nnz, int S, int nslab,
-                                                  const double *__restrict__ z, double *__restrict__ pA,
-                                                  double *__restrict__ pB, double *__restrict__ q,
+                                                  int nrow, int nnz, int S, int nslab, double2 *__restrict__ zpA,
+                                                  double2 *__restrict__ zpB, double *__restrict__ q,
                                                   double *__restrict__ part, const IterState *st_in,
                                                   IterState *st_out, int self_sum)
 {
@@ -323,11 +329,11 @@ __global__ void k_pcg_spmv(const int *__restrict__ cols, const double *__restric
   }
   if (N.done) return;
   const double beta = N.beta;
-  const double *pold = N.pbuf ? pB : pA;
-  double *pnew = N.pbuf ? pA : pB;
+  const double2 *zc = N.pbuf ? zpB : zpA;
+  double2 *zn = N.pbuf ? zpA : zpB;
   const long long row = (long long)blockIdx.x * S + lane;
   double v[1] = {0.0};
-  if (row < nrow) v[0] = pcg_q_row(cols, vals, nnz, S, blockIdx.x, lane, row, z, beta, pold, pnew, q);
+  if (row < nrow) v[0] = pcg_q_row(cols, vals, nnz, S, blockIdx.x, lane, row, zc, beta, zn, q);
   block_sum<1>(v, lds);
   if (lane == 0) part[blockIdx.x * 4] = v[0];
 }
@@ -335,11 +341,9 @@ __global__ void k_pcg_spmv(const int *__restrict__ cols, const double *__restric
 // alpha from the p.q partials; x += alpha p, r -= alpha q, z = dinv r
 // (pcg_update_row); partials r.r, r.z (part[.][1..2]).
 __global__ void k_pcg_update(int nrow, int S, int nslab, const double *__restrict__ dinv,
-                                                    const double *__restrict__ pA, const double *__restrict__ pB,
-                                                    const double *__restrict__ q, double *__restrict__ x,
-                                                    double *__restrict__ r, double *__restrict__ z,
-                                                    double *__restrict__ part, const IterState *st_in,
-                                                    IterState *st_out, int self_sum)
+                             double2 *__restrict__ zpA, double2 *__restrict__ zpB, const double *__restrict__ q,
+                             double *__restrict__ x, double *__restrict__ r, double *__restrict__ part,
+                             const IterState *st_in, IterState *st_out, int self_sum)
 {
   __shared__ double lds[32];
   __shared__ IterState N;
@@ -358,10 +362,10 @@ __global__ void k_pcg_update(int nrow, int S, int nslab, const double *__restric
   if (blockIdx.x == 0 && lane == 0) *st_out = N;
   if (N.done) return;
   const double alpha = N.alpha;
-  const double *p = N.pbuf ? pB : pA;  // the buffer k_pcg_spmv wrote
+  double2 *zp = N.pbuf ? zpB : zpA;  // the pairs whose p k_pcg_spmv wrote
   const long long row = (long long)blockIdx.x * S + lane;
   double v[2] = {0.0, 0.0};
-  if (row < nrow) pcg_update_row(row, alpha, dinv, p, q, x, r, z, v);
+  if (row < nrow) pcg_update_row(row, alpha, dinv, zp, q, x, r, v);
   block_sum<2>(v, lds);
   if (lane == 0) {
     part[blockIdx.x * 4 + 1] = v[0];
@@ -449,9 +453,9 @@ __device__ inline bool grid_barrier(unsigned *bar, unsigned n, int *fault, int &
 
 __global__ __launch_bounds__(kFusedLanes) void k_pcg_fused(
     const int *__restrict__ cols, const double *__restrict__ vals, int nrow, int nnz, int nslab,
-    const double *__restrict__ dinv, double *__restrict__ x, double *__restrict__ r, double *__restrict__ z,
-    double *__restrict__ pA, double *__restrict__ pB, double *__restrict__ q, double *__restrict__ part,
-    IterState *st, unsigned *bar, int its)
+    const double *__restrict__ dinv, double *__restrict__ x, double *__restrict__ r, double2 *__restrict__ zpA,
+    double2 *__restrict__ zpB, double *__restrict__ q, double *__restrict__ part, IterState *st, unsigned *bar,
+    int its)
 {
   constexpr int S = kFusedLanes;
   __shared__ double lds[48];
@@ -468,12 +472,12 @@ __global__ __launch_bounds__(kFusedLanes) void k_pcg_fused(
   unsigned arrivals = 0;
   int k = 0;
   for (; k < its && !done; k++) {
-    const double *pold = pbuf ? pB : pA;
-    double *pnew = pbuf ? pA : pB;
+    const double2 *zc = pbuf ? zpB : zpA;
+    double2 *zn = pbuf ? zpA : zpB;
     for (int slab = blockIdx.x; slab < nslab; slab += G) {
       const long long row = (long long)slab * S + lane;
       double v[1] = {0.0};
-      if (row < nrow) v[0] = pcg_q_row(cols, vals, nnz, S, slab, lane, row, z, beta, pold, pnew, q);
+      if (row < nrow) v[0] = pcg_q_row(cols, vals, nnz, S, slab, lane, row, zc, beta, zn, q);
       block_sum<1>(v, lds);
       if (lane == 0) part[slab * 4] = v[0];
     }
@@ -484,11 +488,11 @@ __global__ __launch_bounds__(kFusedLanes) void k_pcg_fused(
     __syncthreads();
     alpha = pcg_alpha(rz, bc[0]);
     pbuf ^= 1;
-    const double *p = pbuf ? pB : pA;  // = pnew
+    double2 *zp = pbuf ? zpB : zpA;  // = zn
     for (int slab = blockIdx.x; slab < nslab; slab += G) {
       const long long row = (long long)slab * S + lane;
       double v[2] = {0.0, 0.0};
-      if (row < nrow) pcg_update_row(row, alpha, dinv, p, q, x, r, z, v);
+      if (row < nrow) pcg_update_row(row, alpha, dinv, zp, q, x, r, v);
       block_sum<2>(v, lds);
       if (lane == 0) {
         part[slab * 4 + 1] = v[0];
@@ -534,7 +538,9 @@ __global__ void k_zero(double *__restrict__ x, int n)
 
 // PCG work vectors of one solve
 struct PcgVecs {
-  double *r, *z, *pA, *pB, *q, *part;
+  double *r;
+  double2 *zpA, *zpB;  // {z, p} pairs, two buffers
+  double *q, *part;
   IterState *st;   // two copies (the two-launch iteration passes the state between them)
   unsigned *bar;   // grid-barrier words of k_pcg_fused
 };
@@ -546,7 +552,8 @@ struct ell_device_ {
   int device;  // HIP device the mirror lives on
   int *cols;
   double *vals, *dinv;
-  double *x, *b, *r, *z, *pA, *pB, *q, *part;
+  double *x, *b, *r, *q, *part;
+  double2 *zpA, *zpB;  // PCG {z, p} pairs
   IterState *st;
   unsigned *bar;
   int fused_its;   // 0: two launches per PCG iteration; n > 0: k_pcg_fused, n iterations per launch
@@ -648,9 +655,8 @@ extern "C" int ell_device_create(const ell_matrix *m, int slab_rows, ell_device 
   ALLOC(d->x, nvc * sizeof(double));
   ALLOC(d->b, nv * sizeof(double));
   ALLOC(d->r, nv * sizeof(double));
-  ALLOC(d->z, nvc * sizeof(double));
-  ALLOC(d->pA, nvc * sizeof(double));
-  ALLOC(d->pB, nvc * sizeof(double));
+  ALLOC(d->zpA, nvc * sizeof(double2));
+  ALLOC(d->zpB, nvc * sizeof(double2));
   ALLOC(d->q, nv * sizeof(double));
   ALLOC(d->part, (size_t)(d->nslab + 1) * 4 * sizeof(double));
   ALLOC(d->st, 2 * sizeof(IterState));
@@ -793,7 +799,7 @@ static int pcg_run(ell_device *d, const PcgVecs &V, ell_solver *solver, const do
   int rc = set_state(d, V.st, max_its, rtol, 0.0);
   if (rc) return rc;
   k_pcg_init<<<d->nslab, d->S, 0, d->stream>>>(d->cols, d->vals, d->nrow, d->nnz, d->S, x, b, d->dinv, V.r,
-                                               V.z, V.pA, V.pB, V.part);
+                                               V.zpA, V.zpB, V.part);
   k_reduce<<<1, 256, 0, d->stream>>>(V.part, d->nslab, RED_INIT, V.st);
   MM_HIP(hipGetLastError());
   IterState h;
@@ -809,10 +815,11 @@ static int pcg_run(ell_device *d, const PcgVecs &V, ell_solver *solver, const do
     const int *cols = d->cols;
     const double *vals = d->vals, *dinv = d->dinv;
     int nrow = d->nrow, nnz = d->nnz, nslab = d->nslab, its = d->fused_its;
-    double *xv = x, *r = V.r, *z = V.z, *pA = V.pA, *pB = V.pB, *q = V.q, *part = V.part;
+    double *xv = x, *r = V.r, *q = V.q, *part = V.part;
+    double2 *zpA = V.zpA, *zpB = V.zpB;
     IterState *st = V.st;
     unsigned *bar = V.bar;
-    void *args[] = {&cols, &vals, &nrow, &nnz, &nslab, &dinv, &xv, &r, &z, &pA, &pB, &q, &part, &st, &bar, &its};
+    void *args[] = {&cols, &vals, &nrow, &nnz, &nslab, &dinv, &xv, &r, &zpA, &zpB, &q, &part, &st, &bar, &its};
     MM_HIP(hipLaunchCooperativeKernel((const void *)k_pcg_fused, dim3(d->fused_grid), dim3(kFusedLanes), args, 0,
                                       d->stream));
     rc = get_state(d, V.st, &h);
@@ -829,11 +836,11 @@ static int pcg_run(ell_device *d, const PcgVecs &V, ell_solver *solver, const do
   const int self_sum = d->nslab <= kSelfSumSlabs ? 1 : 0;
   while (!h.done) {
     for (int k = 0; k < 32; k++) {
-      k_pcg_spmv<<<d->nslab, d->S, 0, d->stream>>>(d->cols, d->vals, d->nrow, d->nnz, d->S, d->nslab, V.z, V.pA,
-                                                   V.pB, V.q, V.part, V.st, V.st + 1, self_sum);
+      k_pcg_spmv<<<d->nslab, d->S, 0, d->stream>>>(d->cols, d->vals, d->nrow, d->nnz, d->S, d->nslab, V.zpA,
+                                                   V.zpB, V.q, V.part, V.st, V.st + 1, self_sum);
       if (!self_sum) k_reduce<<<1, 256, 0, d->stream>>>(V.part, d->nslab, RED_PQ, V.st + 1);
-      k_pcg_update<<<d->nslab, d->S, 0, d->stream>>>(d->nrow, d->S, d->nslab, d->dinv, V.pA, V.pB, V.q, x, V.r,
-                                                     V.z, V.part, V.st + 1, V.st, self_sum);
+      k_pcg_update<<<d->nslab, d->S, 0, d->stream>>>(d->nrow, d->S, d->nslab, d->dinv, V.zpA, V.zpB, V.q, x,
+                                                     V.r, V.part, V.st + 1, V.st, self_sum);
       if (!self_sum) k_reduce<<<1, 256, 0, d->stream>>>(V.part, d->nslab, RED_RR, V.st);
     }
     MM_HIP(hipGetLastError());
@@ -881,7 +888,7 @@ extern "C" int ell_device_solve_pcg(ell_device *d, ell_solver *solver, const dou
   if (d->nrow == 0) return 0;
   MM_HIP(hipMemcpyAsync(d->x, x, d->nrow * sizeof(double), hipMemcpyHostToDevice, d->stream));
   MM_HIP(hipMemcpyAsync(d->b, b, d->nrow * sizeof(double), hipMemcpyHostToDevice, d->stream));
-  const PcgVecs V{d->r, d->z, d->pA, d->pB, d->q, d->part, d->st, d->bar};
+  const PcgVecs V{d->r, d->zpA, d->zpB, d->q, d->part, d->st, d->bar};
   const int rc = pcg_run(d, V, solver, d->b, d->x);
   if (rc != 0 && rc != MM_NOT_CONVERGED) return rc;
   MM_HIP(hipMemcpyAsync(x, d->x, d->nrow * sizeof(double), hipMemcpyDeviceToHost, d->stream));
@@ -922,9 +929,8 @@ int mm::ell_device_alloc(int nrow, int nnz, int S, ell_device **out)
   ALLOC(d->x, nv * sizeof(double));
   ALLOC(d->b, nv * sizeof(double));
   ALLOC(d->r, nv * sizeof(double));
-  ALLOC(d->z, nv * sizeof(double));
-  ALLOC(d->pA, nv * sizeof(double));
-  ALLOC(d->pB, nv * sizeof(double));
+  ALLOC(d->zpA, nv * sizeof(double2));
+  ALLOC(d->zpB, nv * sizeof(double2));
   ALLOC(d->q, nv * sizeof(double));
   ALLOC(d->part, (size_t)(d->nslab + 1) * 4 * sizeof(double));
   ALLOC(d->st, 2 * sizeof(IterState));
@@ -957,16 +963,16 @@ int mm::ell_device_pcg_device(ell_device *d, ell_solver *solver, const double *b
 {
   if (!d || !b || !x) return MM_ERR_ARG;
   if (d->nrow == 0) return 0;
-  const PcgVecs V{d->r, d->z, d->pA, d->pB, d->q, d->part, d->st, d->bar};
+  const PcgVecs V{d->r, d->zpA, d->zpB, d->q, d->part, d->st, d->bar};
   return pcg_run(d, V, solver, b, x);
 }
 
 // ---- the same solve without any allocation ----------------------------------
 // The PCG work vectors, partial sums and iteration state of
-// ell_device_solve_pcg_noalloc live in this file-scope array: five vectors of
+// ell_device_solve_pcg_noalloc live in this file-scope array: six vectors of
 // ELL_NOALLOC_MAX_ROWS doubles, the partials of up to ELL_NOALLOC_MAX_ROWS / 64
 // slabs, two IterState copies and the fused kernel's barrier words (about
-// 40.5 MiB, held on every GPU the library runs on until the process ends).
+// 48.5 MiB, held on every GPU the library runs on until the process ends).
 // Host code never uses the array's name as an address: the per-device
 // address comes from hipGetSymbolAddress.
 namespace {
@@ -974,7 +980,7 @@ constexpr size_t kNaRows = ELL_NOALLOC_MAX_ROWS;
 constexpr size_t kNaPart = (kNaRows / 64 + 1) * 4;
 constexpr size_t kNaState = (2 * sizeof(IterState) + 7) / 8;
 constexpr size_t kNaBar = kBarBytes / 8;  // the grid-barrier words of k_pcg_fused
-constexpr size_t kNaDoubles = 5 * kNaRows + kNaPart + kNaState + kNaBar;
+constexpr size_t kNaDoubles = 6 * kNaRows + kNaPart + kNaState + kNaBar;
 constexpr int kNaMaxDevices = 64;
 }  // namespace
 
@@ -1019,9 +1025,11 @@ extern "C" int ell_device_solve_pcg_noalloc(ell_device *d, ell_solver *solver, c
   double *sp = nullptr;
   const int rc = noalloc_scratch(d->device, &sp);
   if (rc) return rc;
-  const PcgVecs V{sp, sp + kNaRows, sp + 2 * kNaRows, sp + 3 * kNaRows, sp + 4 * kNaRows, sp + 5 * kNaRows,
-                  reinterpret_cast<IterState *>(sp + 5 * kNaRows + kNaPart),
-                  reinterpret_cast<unsigned *>(sp + 5 * kNaRows + kNaPart + kNaState)};
+  // r, then the two pair buffers (2 rows' worth each, 16-byte aligned), q, the
+  // partials, the two states and the barrier words
+  const PcgVecs V{sp, reinterpret_cast<double2 *>(sp + kNaRows), reinterpret_cast<double2 *>(sp + 3 * kNaRows),
+                  sp + 5 * kNaRows, sp + 6 * kNaRows, reinterpret_cast<IterState *>(sp + 6 * kNaRows + kNaPart),
+                  reinterpret_cast<unsigned *>(sp + 6 * kNaRows + kNaPart + kNaState)};
   return pcg_run(d, V, solver, b, x);
 }
 
@@ -1035,9 +1043,8 @@ extern "C" int ell_device_destroy(ell_device *d)
   (void)hipFree(d->x);
   (void)hipFree(d->b);
   (void)hipFree(d->r);
-  (void)hipFree(d->z);
-  (void)hipFree(d->pA);
-  (void)hipFree(d->pB);
+  (void)hipFree(d->zpA);
+  (void)hipFree(d->zpB);
   (void)hipFree(d->q);
   (void)hipFree(d->part);
   (void)hipFree(d->st);
