@@ -157,18 +157,24 @@ def test_per_rve_totals_same_bits(precision):
     """RVEs of at least big_slabs slabs take their per-iteration dot-product
     totals from kernels R and S (summed once per RVE) instead of summing the
     slab partials in every workgroup; both sum in the same order, so putting
-    every RVE on that path (big_slabs=1) changes no bit.  Uneven RVEs (the
-    march and the slab kernel A), two stream groups, graphs."""
+    every RVE on that path (big_slabs=1), or only the larger ones (a
+    threshold inside the batch's range of slab counts, so that one launch
+    holds RVEs of both kinds), changes no bit.  Uneven RVEs (the march and
+    the slab kernel A), two stream groups, graphs."""
     specs = [configs.c4_spec(5, i) for i in range(5)] + [configs.c2_spec(2, 3, n=20)]
+    nslab = sorted(-(-(s.nex + 1) * (s.ney + 1) * (s.nez + 1) // 256) for s in specs)
+    mid = nslab[len(nslab) // 2]
+    assert nslab[0] < mid <= nslab[-1]
     res = {}
-    for big in (None, 1):
+    for big in (None, 1, mid):
         print("per-RVE totals: precision", precision, "big_slabs", big, flush=True)
         with Engine(nrhs=6, precision=precision, hbm_budget=2 * GB, max_rves=16, big_slabs=big) as eng:
             res[big] = _solve_batch(eng, specs)
-    for a, b in zip(res[None], res[1]):
-        assert a["status"] == 0 and b["status"] == 0
-        assert np.array_equal(a["c_tangent"], b["c_tangent"])
-        assert a["pcg_its"] == b["pcg_its"]
+    for big in (1, mid):
+        for a, b in zip(res[None], res[big]):
+            assert a["status"] == 0 and b["status"] == 0
+            assert np.array_equal(a["c_tangent"], b["c_tangent"])
+            assert a["pcg_its"] == b["pcg_its"]
 
 
 def test_fd_tangent_seven_loads():
