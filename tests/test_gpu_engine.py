@@ -320,6 +320,16 @@ def test_dictionary_storage_equals_full_ell(precision):
         assert a["pcg_its"] == b["pcg_its"] and a["sweeps"] == b["sweeps"]
     assert np.array_equal(out[MM_ELL_FULL, "ell"][0], out[MM_ELL_DICT, "ell"][0])
     assert np.array_equal(out[MM_ELL_FULL, "ell"][1], out[MM_ELL_DICT, "ell"][1])
+    # and both equal the oracle's assembly of that RVE directly: FP64 bit for
+    # bit, FP32 as its rounding
+    ref, _ = oracle_rve(specs[-1])
+    cols_ref, vals_ref = ref.assemble()
+    if precision == MM_PREC_MIXED:
+        vals_ref = vals_ref.astype(np.float32).astype(np.float64)
+    for storage in (MM_ELL_FULL, MM_ELL_DICT):
+        cols, vals = out[storage, "ell"]
+        assert np.array_equal(cols, cols_ref)
+        assert np.array_equal(vals.view(np.uint64), vals_ref.view(np.uint64))
 
 
 @pytest.mark.parametrize("storage", [MM_ELL_FULL, MM_ELL_DICT])
