@@ -206,9 +206,9 @@ typedef struct {
                             neighbour from memory; n > 0 (default 8) marches
                             in-plane chunks of slab_nodes nodes through runs
                             of n z-planes with the neighbour planes staged in
-                            LDS (MM_PREC_MIXED with nrhs >= 2 only; RVEs
-                            whose x-extent exceeds slab_nodes/2 - 1 elements
-                            stay slab by slab).  Same SpMV bits; the p.q
+                            LDS (nrhs >= 2 only: two load cases per tile in
+                            FP32, one in FP64; RVEs whose x-extent exceeds
+                            slab_nodes/2 - 1 elements stay slab by slab).  Same SpMV bits; the p.q
                             partial sums are grouped differently. */
   int big_slabs;         /* RVEs of at least this many slabs (default 1024)
                             have their per-iteration dot products summed once

@@ -680,8 +680,9 @@ static void warm_coefficients(const double (*old)[MM_NVOI], int nold, const doub
 // Plane-march tiling of an RVE of nx x ny x nz nodes in nslab slabs (kernel
 // A's tiles, RveDev::mcpp ..): in-plane chunks of at most SN nodes times
 // runs of at most march_planes z-planes (even splits) times groups of load
-// cases, the run doubled until the tiles fit the RVE's slabs.  mcpp = 0 (slab by slab) when the march is
-// off, in FP64 engines, with one load case (27 gathers per node cost less
+// cases (two in FP32, one in FP64, whose ring takes twice the bytes), the
+// run doubled until the tiles fit the RVE's slabs.  mcpp = 0 (slab by slab)
+// when the march is off, with one load case (27 gathers per node cost less
 // than a tile's fixed work), or when the window (SN + 2 nx + 2 nodes) is
 // longer than two positions per lane or the march's LDS exceeds its bound.  Fixed by
 // the RVE and the engine configuration alone, so an RVE's bits do not depend
@@ -695,7 +696,7 @@ static MarchTiling march_tiling(const mm_engine *e, const mm_rve_spec &s, long l
   MarchTiling m;
   const int nx = s.nex + 1, ny = s.ney + 1, nz = s.nez + 1, SN = e->SN;
   if (e->cfg.march_planes <= 0 || 2 * nx + 2 > SN) return m;
-  if (e->fp64 || e->K < 2 || march_lds_bytes(SN, nx, ny, e->K, (int)e->tsz, e->dict_mode) > kMarchMaxLds)
+  if (e->K < 2 || march_lds_bytes(SN, nx, ny, e->K, (int)e->tsz, e->dict_mode) > kMarchMaxLds)
     return m;
   const long long cpp = ((long long)nx * ny + SN - 1) / SN;
   long long L = e->cfg.march_planes;
@@ -955,7 +956,7 @@ static int update_active(mm_engine *e)
   std::sort(runs.begin(), runs.end());
   const size_t stride = (size_t)e->nslab_cap + 1;
   const int ngroup = e->stream2 ? 2 : 1;
-  const bool march = e->cfg.march_planes > 0 && !e->fp64;
+  const bool march = e->cfg.march_planes > 0;
   long long n[2] = {0, 0}, nt[2] = {0, 0};
   int lds[2] = {0, 0}, slab_mode[2] = {0, 0}, big[2] = {0, 0};
   int g = 0;
@@ -1446,7 +1447,7 @@ static int time_kernel(mm_engine *e, int what, int reps, double *avg_ms, double 
       lds = std::max(lds, need);
       if (need == 0) slab_mode = 1;
     }
-  if (e->cfg.march_planes > 0 && !e->fp64 && lds > 0) {
+  if (e->cfg.march_planes > 0 && lds > 0) {
     P.a_lds = march_lds_class(lds);
     P.a_slab = slab_mode;
   }

@@ -2020,11 +2020,9 @@ static int launch_steps(const EngPtrs &P, const SolveParams &sp, int grid, int f
       case 0:
         // with the march on (a_lds > 0), its tiles and, if any RVE is
         // processed slab by slab (a_slab), k_step_a for those
-        if constexpr (sizeof(TS) == 4) {  // the march is FP32 only
-          if (P.a_lds > 0 && P.a_grid > 0) {
-            if (DICT) k_step_a_march_dict<TS, K><<<P.a_grid, P.SN, P.a_lds, s>>>(P, sp);
-            else k_step_a_march_full<TS, K><<<P.a_grid, P.SN, P.a_lds, s>>>(P, sp);
-          }
+        if (P.a_lds > 0 && P.a_grid > 0) {
+          if (DICT) k_step_a_march_dict<TS, K><<<P.a_grid, P.SN, P.a_lds, s>>>(P, sp);
+          else k_step_a_march_full<TS, K><<<P.a_grid, P.SN, P.a_lds, s>>>(P, sp);
         }
         if (P.a_lds == 0 || P.a_slab) k_step_a<TS, K, DICT><<<grid, P.SN, 0, s>>>(P, sp);
         if (P.s_big) k_step_s<TS, K><<<grid < kRGrid ? grid : kRGrid, P.SN, 0, s>>>(P);
@@ -2053,8 +2051,7 @@ template <typename TS, int K, bool DICT>
 static int launch_time(const EngPtrs &P, int grid, int what, hipStream_t s)
 {
   if (what == 1) {
-    if constexpr (sizeof(TS) == 4)
-      if (P.a_lds > 0) k_march_time<TS, K, DICT><<<grid, P.SN, P.a_lds, s>>>(P);
+    if (P.a_lds > 0) k_march_time<TS, K, DICT><<<grid, P.SN, P.a_lds, s>>>(P);
     if (P.a_lds == 0 || P.a_slab) k_inner_time<TS, K, DICT><<<grid, P.SN, 0, s>>>(P);
   } else {
     k_spmv_time<TS, K, DICT><<<grid, P.SN, 0, s>>>(P);

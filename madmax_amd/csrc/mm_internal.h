@@ -195,7 +195,7 @@ constexpr int kDictEntry = 27 * kDictBlk;
 // Load cases staged in LDS per round of the plane march.  Two keep an FP32
 // march of a 30^3 RVE (window 318 nodes) with its dictionary cache under
 // 40 KB, four workgroups per CU; three cases per round fit only three.
-constexpr int march_cases(int) { return 2; }
+constexpr int march_cases(int tsz) { return tsz == 8 ? 1 : 2; }
 // Default of mm_engine_config::big_slabs: RVEs of at least this many slabs
 // get their dot-product totals from kernels R and S (RveDev::big); below it
 // the re-summing in every workgroup costs less than the extra latency of

@@ -97,9 +97,11 @@ def test_slab_shapes_agree():
         assert normwise_rel(b["c_tangent"], c["c_tangent"]) <= 1e-12
 
 
-def test_plane_march_matches_slab_kernel():
+@pytest.mark.parametrize("precision", [MM_PREC_MIXED, MM_PREC_FP64])
+def test_plane_march_matches_slab_kernel(precision):
     """Kernel A slab by slab (march_planes=0) and as a z-plane march with two
-    run lengths: every variant meets the oracle bar and they agree closely
+    run lengths, in FP32 inside FP64 refinement (two load cases per tile) and
+    in FP64 (one): every variant meets the oracle bar and they agree closely
     (the SpMV bits are the same; only the grouping of the p.q partial sums
     differs)."""
     specs = [rve_spec(8, [(0.45, 0.5, 0.3)]), rve_spec((12, 9, 7), [(0.4, 0.6, 0.25)]),
@@ -107,7 +109,9 @@ def test_plane_march_matches_slab_kernel():
     refs = [_oracle_tangent(s) for s in specs]
     out = {}
     for planes in (0, 8, 2):
-        with Engine(nrhs=6, hbm_budget=GB, max_rves=16, march_planes=planes) as eng:
+        print("march planes", planes, "precision", precision, flush=True)
+        with Engine(nrhs=6, precision=precision, hbm_budget=GB, max_rves=16, march_planes=planes,
+                    inner_rtol=1e-10 if precision == MM_PREC_FP64 else 1e-5) as eng:
             out[planes] = _solve_batch(eng, specs)
         for s, r, c_ref in zip(specs, out[planes], refs):
             assert r["status"] == 0
